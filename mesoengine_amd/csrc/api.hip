@@ -1,0 +1,570 @@
+// C-ABI entry points: errors, context, buffers, streams, events, scenes,
+// raymarch.  Kernels live in raymarch.hip / mesh.hip / voxelize.hip.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "meso_internal.h"
+
+namespace meso {
+
+static thread_local std::string g_error;
+
+void set_error(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_error = buf;
+}
+
+meso_status fail(meso_status code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+hipStream_t stream_of(meso_context *ctx, meso_stream s) {
+  return s ? (hipStream_t)s : ctx->stream;
+}
+
+__global__ void zero_kernel(uint4 *p, uint64_t n16, uint8_t *tail, uint32_t ntail) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+    p[i] = make_uint4(0, 0, 0, 0);
+  if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
+// Clear device memory with a kernel rather than hipMemsetAsync, so that
+// callers may capture it into a graph.  ptr must be 16-byte aligned.
+meso_status launch_zero(void *ptr, uint64_t bytes, hipStream_t stream) {
+  if (!bytes) return MESO_OK;
+  MESO_CHECK(((uintptr_t)ptr & 15u) == 0, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "zero-fill pointer must be 16-byte aligned");
+  const uint64_t n16 = bytes / 16;
+  const uint32_t ntail = (uint32_t)(bytes % 16);
+  uint64_t blocks = (n16 + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(zero_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, (uint4 *)ptr, n16,
+                     (uint8_t *)ptr + n16 * 16, ntail);
+  MESO_HIP(hipGetLastError());
+  return MESO_OK;
+}
+
+// ------------------------------------------------------------------ RNG
+uint64_t splitmix64(uint64_t *s) {
+  uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+uint32_t Pcg32::next() {
+  uint64_t old = state;
+  state = old * 6364136223846793005ull + inc;
+  uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+  uint32_t rot = (uint32_t)(old >> 59u);
+  return (xs >> rot) | (xs << ((0u - rot) & 31u));
+}
+void Pcg32::srandom(uint64_t initstate, uint64_t initseq) {
+  state = 0;
+  inc = (initseq << 1u) | 1u;
+  next();
+  state += initstate;
+  next();
+}
+void Pcg32::seed(uint64_t s) {
+  uint64_t sm = s;
+  uint64_t a = splitmix64(&sm);
+  uint64_t b = splitmix64(&sm);
+  srandom(a, b);
+}
+
+// ------------------------------------------------------------------ camera
+static void normalize3(float v[3]) {
+  float xx = v[0] * v[0], yy = v[1] * v[1], zz = v[2] * v[2];
+  float s = xx + yy;
+  s = s + zz;
+  float len = std::sqrt(s);
+  v[0] = v[0] / len;
+  v[1] = v[1] / len;
+  v[2] = v[2] / len;
+}
+static void cross3(const float a[3], const float b[3], float out[3]) {
+  float t0 = a[1] * b[2], t1 = a[2] * b[1];
+  float t2 = a[2] * b[0], t3 = a[0] * b[2];
+  float t4 = a[0] * b[1], t5 = a[1] * b[0];
+  out[0] = t0 - t1;
+  out[1] = t2 - t3;
+  out[2] = t4 - t5;
+}
+
+// Vertical-FOV pinhole: FVoxelCamera::GetProjectionMatrix uses
+// glm::perspective with the vertical fov (Runtimes/Instance/VoxelCamera.cpp:12-17);
+// forward is eye->target (VoxelCamera.cpp:24-29 takes -row 2 of the view).
+void derive_camera(const meso_camera *c, DevCamera *o) {
+  for (int i = 0; i < 3; i++) {
+    o->eye[i] = c->eye[i];
+    o->fwd[i] = c->target[i] - c->eye[i];
+  }
+  normalize3(o->fwd);
+  cross3(o->fwd, c->up, o->right);
+  normalize3(o->right);
+  cross3(o->right, o->fwd, o->up);
+  float th = (float)std::tan((double)c->vfov_deg * 3.14159265358979323846 / 360.0);
+  float aspect = (float)c->width / (float)c->height;
+  o->sx = th * aspect;
+  o->sy = th;
+  o->inv_w2 = 2.0f / (float)c->width;
+  o->inv_h2 = 2.0f / (float)c->height;
+  o->width = c->width;
+  o->height = c->height;
+}
+
+void derive_shading(const meso_shading *s, DevShading *o) {
+  float l[3] = {s->light[0], s->light[1], s->light[2]};
+  normalize3(l);
+  for (int i = 0; i < 3; i++) o->light[i] = l[i];
+  o->ao_seed = s->ao_seed;
+  o->ao_samples = s->ao_samples;
+  o->ao_radius = s->ao_radius;
+}
+
+static meso_status check_camera(const meso_camera *cam) {
+  MESO_CHECK(cam, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "camera is NULL");
+  MESO_CHECK(cam->size >= sizeof(meso_camera), MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "meso_camera.size %u < %zu", cam->size, sizeof(meso_camera));
+  MESO_CHECK(cam->width > 0 && cam->height > 0 && cam->width <= 16384 && cam->height <= 16384,
+             MESO_ERR_ARGUMENT_OUT_OF_RANGE, "image %dx%d outside 1..16384", cam->width,
+             cam->height);
+  MESO_CHECK(cam->vfov_deg > 0.0f && cam->vfov_deg < 180.0f, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "vfov %f outside (0,180)", (double)cam->vfov_deg);
+  float f[3] = {cam->target[0] - cam->eye[0], cam->target[1] - cam->eye[1],
+                cam->target[2] - cam->eye[2]};
+  float c[3];
+  cross3(f, cam->up, c);
+  MESO_CHECK(c[0] != 0.0f || c[1] != 0.0f || c[2] != 0.0f, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "camera up is parallel to the view direction (or eye == target)");
+  return MESO_OK;
+}
+
+}  // namespace meso
+
+using namespace meso;
+
+// ------------------------------------------------------------------ ABI
+extern "C" {
+
+const char *meso_last_error_message(void) { return g_error.c_str(); }
+uint32_t meso_abi_version(void) { return MESO_ABI_VERSION; }
+
+meso_status meso_context_create(int32_t device, meso_context **out) {
+  MESO_CHECK(out, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "out is NULL");
+  *out = nullptr;
+  int count = 0;
+  MESO_HIP(hipGetDeviceCount(&count));
+  MESO_CHECK(device >= 0 && device < count, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "device %d not in [0,%d)", device, count);
+  MESO_HIP(hipSetDevice(device));
+  auto *ctx = new meso_context();
+  ctx->device = device;
+  hipError_t e = hipGetDeviceProperties(&ctx->props, device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return fail(MESO_ERR_RUNTIME, "context setup failed: %s", hipGetErrorString(e));
+  }
+  *out = ctx;
+  return MESO_OK;
+}
+
+meso_status meso_context_destroy(meso_context *ctx) {
+  if (!ctx) return MESO_OK;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) {
+    hipStreamSynchronize(ctx->stream);
+    hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+  return MESO_OK;
+}
+
+meso_status meso_device_info_get(meso_context *ctx, meso_device_info *out) {
+  MESO_CHECK(ctx && out, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  std::memset(out, 0, sizeof(*out));
+  out->size = sizeof(*out);
+  std::snprintf(out->name, sizeof(out->name), "%s", ctx->props.gcnArchName);
+  out->compute_units = ctx->props.multiProcessorCount;
+  out->xcds = 8;
+  out->lds_bytes_per_cu = (int32_t)ctx->props.maxSharedMemoryPerMultiProcessor;
+  out->l2_bytes = ctx->props.l2CacheSize;
+  out->hbm_bytes = ctx->props.totalGlobalMem;
+  out->clock_khz = ctx->props.clockRate;
+  out->wavefront = ctx->props.warpSize;
+  return MESO_OK;
+}
+
+// ---------------- buffers
+meso_status meso_buffer_create(meso_context *ctx, uint64_t size, const void *init,
+                               meso_buffer **out) {
+  MESO_CHECK(ctx && out, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  *out = nullptr;
+  MESO_CHECK(size > 0, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "buffer size is 0");
+  MESO_HIP(hipSetDevice(ctx->device));
+  void *p = nullptr;
+  MESO_HIP(hipMalloc(&p, size));
+  if (init) {
+    hipError_t e = hipMemcpy(p, init, size, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hipFree(p);
+      return fail(MESO_ERR_RUNTIME, "initial upload failed: %s", hipGetErrorString(e));
+    }
+  }
+  auto *b = new meso_buffer();
+  b->ctx = ctx;
+  b->ptr = p;
+  b->size = size;
+  *out = b;
+  return MESO_OK;
+}
+
+meso_status meso_buffer_destroy(meso_buffer *buf) {
+  if (!buf) return MESO_OK;
+  hipSetDevice(buf->ctx->device);
+  hipFree(buf->ptr);
+  delete buf;
+  return MESO_OK;
+}
+
+meso_status meso_buffer_upload(meso_buffer *buf, const void *src, uint64_t size, uint64_t offset) {
+  MESO_CHECK(buf && src, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  MESO_CHECK(offset <= buf->size && size <= buf->size - offset, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "upload [%llu,+%llu) outside buffer of %llu bytes", (unsigned long long)offset,
+             (unsigned long long)size, (unsigned long long)buf->size);
+  MESO_HIP(hipMemcpy((char *)buf->ptr + offset, src, size, hipMemcpyHostToDevice));
+  return MESO_OK;
+}
+
+meso_status meso_buffer_download(meso_buffer *buf, void *dst, uint64_t size, uint64_t offset) {
+  MESO_CHECK(buf && dst, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  MESO_CHECK(offset <= buf->size && size <= buf->size - offset, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "download [%llu,+%llu) outside buffer of %llu bytes", (unsigned long long)offset,
+             (unsigned long long)size, (unsigned long long)buf->size);
+  MESO_HIP(hipMemcpy(dst, (const char *)buf->ptr + offset, size, hipMemcpyDeviceToHost));
+  return MESO_OK;
+}
+
+uint64_t meso_buffer_device_address(meso_buffer *buf, uint64_t offset) {
+  if (!buf || offset >= buf->size || (offset & 7u)) {
+    set_error("device address offset invalid or not 8-byte aligned");
+    return 0;
+  }
+  return (uint64_t)(uintptr_t)buf->ptr + offset;
+}
+
+uint64_t meso_buffer_size(meso_buffer *buf) { return buf ? buf->size : 0; }
+
+meso_status meso_buffer_fill_zero(meso_buffer *buf, meso_stream stream) {
+  MESO_CHECK(buf, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL buffer");
+  return launch_zero(buf->ptr, buf->size, stream_of(buf->ctx, stream));
+}
+
+// ---------------- streams / events
+meso_status meso_stream_create(meso_context *ctx, meso_stream *out) {
+  MESO_CHECK(ctx && out, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  hipStream_t s;
+  MESO_HIP(hipSetDevice(ctx->device));
+  MESO_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *out = (meso_stream)s;
+  return MESO_OK;
+}
+meso_status meso_stream_destroy(meso_context *ctx, meso_stream stream) {
+  MESO_CHECK(ctx && stream, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  MESO_HIP(hipStreamDestroy((hipStream_t)stream));
+  return MESO_OK;
+}
+meso_status meso_stream_sync(meso_context *ctx, meso_stream stream) {
+  MESO_CHECK(ctx, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL context");
+  MESO_HIP(hipStreamSynchronize(stream_of(ctx, stream)));
+  return MESO_OK;
+}
+meso_status meso_event_create(meso_context *ctx, meso_event **out) {
+  MESO_CHECK(ctx && out, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  hipEvent_t e;
+  MESO_HIP(hipEventCreate(&e));
+  auto *ev = new meso_event();
+  ev->ctx = ctx;
+  ev->ev = e;
+  *out = ev;
+  return MESO_OK;
+}
+meso_status meso_event_destroy(meso_event *ev) {
+  if (!ev) return MESO_OK;
+  hipEventDestroy(ev->ev);
+  delete ev;
+  return MESO_OK;
+}
+meso_status meso_event_record(meso_event *ev, meso_stream stream) {
+  MESO_CHECK(ev, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL event");
+  MESO_HIP(hipEventRecord(ev->ev, stream_of(ev->ctx, stream)));
+  return MESO_OK;
+}
+meso_status meso_event_sync(meso_event *ev) {
+  MESO_CHECK(ev, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL event");
+  MESO_HIP(hipEventSynchronize(ev->ev));
+  return MESO_OK;
+}
+meso_status meso_event_elapsed_ms(meso_event *a, meso_event *b, float *ms) {
+  MESO_CHECK(a && b && ms, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  MESO_HIP(hipEventElapsedTime(ms, a->ev, b->ev));
+  return MESO_OK;
+}
+meso_status meso_memcpy_d2h(meso_context *ctx, void *dst, const void *src, uint64_t bytes,
+                            meso_stream stream) {
+  MESO_CHECK(ctx && dst && src, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  hipStream_t s = stream_of(ctx, stream);
+  MESO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+  MESO_HIP(hipStreamSynchronize(s));
+  return MESO_OK;
+}
+meso_status meso_memcpy_h2d(meso_context *ctx, void *dst, const void *src, uint64_t bytes,
+                            meso_stream stream) {
+  MESO_CHECK(ctx && dst && src, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  hipStream_t s = stream_of(ctx, stream);
+  MESO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+  MESO_HIP(hipStreamSynchronize(s));
+  return MESO_OK;
+}
+
+// ---------------- scenes
+static meso_status scene_alloc(meso_context *ctx, void **p, uint64_t bytes, const void *src,
+                               meso_scene *sc) {
+  if (!bytes) {
+    *p = nullptr;
+    return MESO_OK;
+  }
+  MESO_HIP(hipMalloc(p, bytes));
+  sc->bytes += bytes;
+  if (src) MESO_HIP(hipMemcpy(*p, src, bytes, hipMemcpyHostToDevice));
+  return MESO_OK;
+}
+
+static void scene_free(meso_scene *sc) {
+  if (!sc) return;
+  hipSetDevice(sc->ctx->device);
+  if (sc->dense) hipFree(sc->dense);
+  if (sc->slots) hipFree(sc->slots);
+  if (sc->masks) hipFree(sc->masks);
+  if (sc->voxels) hipFree(sc->voxels);
+  if (sc->top) hipFree(sc->top);
+  delete sc;
+}
+
+#define SCENE_TRY(expr)        \
+  do {                         \
+    meso_status st_ = (expr);  \
+    if (st_ != MESO_OK) {      \
+      scene_free(sc);          \
+      return st_;              \
+    }                          \
+  } while (0)
+
+meso_status meso_scene_create_dense_u8(meso_context *ctx, int32_t n, const uint8_t *voxels,
+                                       meso_scene **out) {
+  MESO_CHECK(ctx && voxels && out, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  MESO_CHECK(n >= 1 && n <= 2048, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "dense edge %d not in 1..2048", n);
+  MESO_HIP(hipSetDevice(ctx->device));
+  auto *sc = new meso_scene();
+  sc->ctx = ctx;
+  sc->kind = MESO_SCENE_DENSE_U8;
+  sc->n = n;
+  SCENE_TRY(scene_alloc(ctx, (void **)&sc->dense, (uint64_t)n * n * n, voxels, sc));
+  *out = sc;
+  return MESO_OK;
+}
+
+meso_status meso_scene_create_brickmap_u16(meso_context *ctx, int32_t n, const uint32_t *slots,
+                                           int64_t n_bricks, const uint64_t *masks,
+                                           const uint16_t *voxels, meso_scene **out) {
+  MESO_CHECK(ctx && slots && out, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  MESO_CHECK(n >= 8 && n % 8 == 0 && n <= 8192, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "brick map edge %d must be a multiple of 8 in 8..8192", n);
+  MESO_CHECK(n_bricks >= 0 && n_bricks < (int64_t)MESO_EMPTY_SLOT, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "n_bricks %lld out of range", (long long)n_bricks);
+  MESO_CHECK(n_bricks == 0 || (masks && voxels), MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "masks/voxels NULL with %lld bricks", (long long)n_bricks);
+  const int64_t nb = n / 8, ncell = nb * nb * nb;
+  // every slot must name an existing brick: the kernels index the pools with it
+  for (int64_t i = 0; i < ncell; i++)
+    MESO_CHECK(slots[i] == MESO_EMPTY_SLOT || (int64_t)slots[i] < n_bricks,
+               MESO_ERR_ARGUMENT_OUT_OF_RANGE, "slot[%lld] = %u >= n_bricks %lld", (long long)i,
+               slots[i], (long long)n_bricks);
+  MESO_HIP(hipSetDevice(ctx->device));
+  auto *sc = new meso_scene();
+  sc->ctx = ctx;
+  sc->kind = MESO_SCENE_BRICKMAP_U16;
+  sc->n = n;
+  sc->n_bricks = n_bricks;
+  SCENE_TRY(scene_alloc(ctx, (void **)&sc->slots, (uint64_t)ncell * 4, slots, sc));
+  SCENE_TRY(scene_alloc(ctx, (void **)&sc->masks, (uint64_t)n_bricks * 64, masks, sc));
+  SCENE_TRY(scene_alloc(ctx, (void **)&sc->voxels, (uint64_t)n_bricks * 1024, voxels, sc));
+  *out = sc;
+  return MESO_OK;
+}
+
+meso_status meso_scene_create_two_level_u16(meso_context *ctx, int32_t n, const uint32_t *top,
+                                            int64_t n_nodes, const uint32_t *node_slots,
+                                            int64_t n_bricks, const uint64_t *masks,
+                                            const uint16_t *voxels, meso_scene **out) {
+  MESO_CHECK(ctx && top && out, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  MESO_CHECK(n >= 128 && n % 128 == 0 && n <= 16384, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "two-level edge %d must be a multiple of 128 in 128..16384", n);
+  MESO_CHECK(n_nodes >= 0 && n_bricks >= 0 && n_bricks < (int64_t)MESO_EMPTY_SLOT,
+             MESO_ERR_ARGUMENT_OUT_OF_RANGE, "negative or oversized counts");
+  MESO_CHECK(n_nodes == 0 || node_slots, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "node_slots NULL");
+  MESO_CHECK(n_bricks == 0 || (masks && voxels), MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "masks/voxels NULL");
+  const int64_t nt = n / 128, ntop = nt * nt * nt;
+  for (int64_t i = 0; i < ntop; i++)
+    MESO_CHECK(top[i] == MESO_EMPTY_SLOT || (int64_t)top[i] < n_nodes,
+               MESO_ERR_ARGUMENT_OUT_OF_RANGE, "top[%lld] = %u >= n_nodes", (long long)i, top[i]);
+  for (int64_t i = 0; i < n_nodes * 4096; i++)
+    MESO_CHECK(node_slots[i] == MESO_EMPTY_SLOT || (int64_t)node_slots[i] < n_bricks,
+               MESO_ERR_ARGUMENT_OUT_OF_RANGE, "node slot %lld = %u >= n_bricks", (long long)i,
+               node_slots[i]);
+  MESO_HIP(hipSetDevice(ctx->device));
+  auto *sc = new meso_scene();
+  sc->ctx = ctx;
+  sc->kind = MESO_SCENE_TWO_LEVEL_U16;
+  sc->n = n;
+  sc->n_nodes = n_nodes;
+  sc->n_bricks = n_bricks;
+  SCENE_TRY(scene_alloc(ctx, (void **)&sc->top, (uint64_t)ntop * 4, top, sc));
+  SCENE_TRY(scene_alloc(ctx, (void **)&sc->slots, (uint64_t)n_nodes * 4096 * 4, node_slots, sc));
+  SCENE_TRY(scene_alloc(ctx, (void **)&sc->masks, (uint64_t)n_bricks * 64, masks, sc));
+  SCENE_TRY(scene_alloc(ctx, (void **)&sc->voxels, (uint64_t)n_bricks * 1024, voxels, sc));
+  *out = sc;
+  return MESO_OK;
+}
+
+meso_status meso_scene_get_info(meso_scene *sc, meso_scene_info *out) {
+  MESO_CHECK(sc && out, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  out->size = sizeof(*out);
+  out->kind = sc->kind;
+  out->n = sc->n;
+  out->n_bricks = sc->n_bricks;
+  out->n_nodes = sc->n_nodes;
+  out->device_bytes = sc->bytes;
+  out->solid_voxels = sc->solid;
+  return MESO_OK;
+}
+
+meso_status meso_scene_download(meso_scene *sc, int32_t part, void *dst, uint64_t bytes) {
+  MESO_CHECK(sc && dst, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  const void *src = nullptr;
+  uint64_t have = 0;
+  const uint64_t n = (uint64_t)sc->n;
+  switch (part) {
+    case MESO_PART_DENSE:
+      src = sc->dense;
+      have = sc->dense ? n * n * n : 0;
+      break;
+    case MESO_PART_SLOTS:
+      src = sc->slots;
+      have = sc->kind == MESO_SCENE_BRICKMAP_U16 ? (n / 8) * (n / 8) * (n / 8) * 4
+                                                 : (uint64_t)sc->n_nodes * 4096 * 4;
+      break;
+    case MESO_PART_MASKS:
+      src = sc->masks;
+      have = (uint64_t)sc->n_bricks * 64;
+      break;
+    case MESO_PART_VOXELS:
+      src = sc->voxels;
+      have = (uint64_t)sc->n_bricks * 1024;
+      break;
+    case MESO_PART_TOP:
+      src = sc->top;
+      have = sc->top ? (n / 128) * (n / 128) * (n / 128) * 4 : 0;
+      break;
+    default: return fail(MESO_ERR_ARGUMENT_OUT_OF_RANGE, "unknown scene part %d", part);
+  }
+  MESO_CHECK(src && bytes <= have, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "part %d holds %llu bytes, %llu requested", part, (unsigned long long)have,
+             (unsigned long long)bytes);
+  MESO_HIP(hipSetDevice(sc->ctx->device));
+  MESO_HIP(hipStreamSynchronize(sc->ctx->stream));
+  MESO_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return MESO_OK;
+}
+
+meso_status meso_scene_destroy(meso_scene *sc) {
+  if (!sc) return MESO_OK;
+  hipStreamSynchronize(sc->ctx->stream);
+  scene_free(sc);
+  return MESO_OK;
+}
+
+// ---------------- raymarch
+meso_status meso_raymarch(meso_context *ctx, meso_scene *scene, const meso_camera *cam,
+                          uint32_t flags, const meso_shading *shading, const meso_ray_outputs *out,
+                          meso_stream stream) {
+  MESO_CHECK(ctx && scene && out, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  MESO_CHECK(scene->ctx == ctx, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "scene belongs to another context");
+  meso_status st = check_camera(cam);
+  if (st != MESO_OK) return st;
+  MESO_CHECK(out->size >= sizeof(meso_ray_outputs), MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "meso_ray_outputs.size too small");
+  MESO_CHECK(out->hit_index, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "hit_index is NULL");
+  MESO_CHECK(out->index_bytes == 4 || out->index_bytes == 8, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "index_bytes must be 4 or 8");
+  const uint64_t nvox = (uint64_t)scene->n * scene->n * scene->n;
+  MESO_CHECK(out->index_bytes == 8 || nvox <= 0x7FFFFFFFull, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "scene of %llu voxels needs 8-byte hit indices", (unsigned long long)nvox);
+  MESO_CHECK((flags & ~(MESO_RAY_SHADOW | MESO_RAY_AO | MESO_RAY_COUNT)) == 0,
+             MESO_ERR_ARGUMENT_OUT_OF_RANGE, "unknown ray flags 0x%x", flags);
+  MESO_CHECK(!(flags & MESO_RAY_COUNT) || out->counters, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "MESO_RAY_COUNT needs a counters buffer");
+  DevShading sh{};
+  if (flags & (MESO_RAY_SHADOW | MESO_RAY_AO)) {
+    MESO_CHECK(shading && shading->size >= sizeof(meso_shading), MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+               "shadow/AO rays need a meso_shading");
+    MESO_CHECK(shading->light[0] != 0.0f || shading->light[1] != 0.0f || shading->light[2] != 0.0f,
+               MESO_ERR_ARGUMENT_OUT_OF_RANGE, "light direction is zero");
+    if (flags & MESO_RAY_AO)
+      MESO_CHECK(shading->ao_samples >= 1 && shading->ao_samples <= 8 && shading->ao_radius > 0.0f,
+                 MESO_ERR_ARGUMENT_OUT_OF_RANGE, "AO needs 1..8 samples and a positive radius");
+    derive_shading(shading, &sh);
+  } else if (shading && shading->size >= sizeof(meso_shading) &&
+             (shading->light[0] != 0.0f || shading->light[1] != 0.0f || shading->light[2] != 0.0f)) {
+    derive_shading(shading, &sh);
+  } else {
+    sh.light[1] = 1.0f;
+  }
+  DevCamera dc;
+  derive_camera(cam, &dc);
+  RayOut ro{out->hit_index, out->index_bytes, out->face, out->t, out->shade,
+            (unsigned long long *)out->counters};
+  MESO_HIP(hipSetDevice(ctx->device));
+  const RayLaunch *shape = nullptr;
+  RayLaunch tuned;
+  {
+    std::lock_guard<std::mutex> lk(ctx->tune_mu);
+    char key[128];
+    std::snprintf(key, sizeof(key), "ray:%d:%d:%dx%d:%u", scene->kind, scene->n, cam->width,
+                  cam->height, flags & ~MESO_RAY_COUNT);
+    auto it = ctx->tune.find(key);
+    if (it != ctx->tune.end()) {
+      tuned = RayLaunch{it->second.variant, it->second.block, it->second.tile_w, it->second.tile_h};
+      shape = &tuned;
+    }
+  }
+  return launch_raymarch(ctx, scene, dc, flags, sh, ro, stream_of(ctx, stream), shape);
+}
+
+}  // extern "C"

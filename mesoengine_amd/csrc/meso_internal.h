@@ -1,0 +1,152 @@
+// Internal declarations of the gfx950 voxel library (not part of the C-ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "../../include/meso_hip.h"
+
+namespace meso {
+
+constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;
+
+// ---------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+meso_status fail(meso_status code, const char *fmt, ...);
+
+#define MESO_HIP(call)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return ::meso::fail(e_ == hipErrorOutOfMemory ? MESO_ERR_OUT_OF_MEMORY        \
+                                                    : MESO_ERR_RUNTIME,             \
+                          "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),    \
+                          __FILE__, __LINE__);                                      \
+  } while (0)
+
+#define MESO_CHECK(cond, code, ...)                                                 \
+  do {                                                                              \
+    if (!(cond)) return ::meso::fail(code, __VA_ARGS__);                            \
+  } while (0)
+
+// ---------------------------------------------------------------- device structs
+// Pinhole camera derived on the host (same operation order as the oracle).
+struct DevCamera {
+  float eye[3], fwd[3], right[3], up[3];
+  float sx, sy, inv_w2, inv_h2;
+  int32_t width, height;
+};
+
+struct DevShading {
+  float light[3];
+  uint32_t ao_seed;
+  int32_t ao_samples;
+  float ao_radius;
+};
+
+struct DevScene {
+  int32_t n;
+  const uint8_t *dense;
+  const uint32_t *slots;
+  const uint64_t *masks;
+  const uint16_t *voxels;
+  const uint32_t *top;
+};
+
+struct RayOut {
+  void *index;
+  int32_t index_bytes;
+  uint8_t *face;
+  float *t;
+  float *shade;
+  unsigned long long *counters;
+};
+
+// raymarch launch shape (autotuner variants)
+struct RayLaunch {
+  int32_t variant;  // 0: 8x8 wave packets in a 16x16 tile; 1: 8x8 packets, 32x8 tile; ...
+  int32_t block;
+  int32_t tile_w, tile_h;
+};
+
+struct MeshLaunch {
+  int32_t block;        // threads per workgroup
+  int32_t grid;         // persistent workgroups (0 = derive from occupancy)
+};
+
+}  // namespace meso
+
+// ---------------------------------------------------------------- handles
+struct meso_context {
+  int32_t device = 0;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t props{};
+  std::mutex tune_mu;
+  std::map<std::string, meso_tune_result> tune;
+};
+
+struct meso_buffer {
+  meso_context *ctx = nullptr;
+  void *ptr = nullptr;
+  uint64_t size = 0;
+};
+
+struct meso_scene {
+  meso_context *ctx = nullptr;
+  int32_t kind = 0;
+  int32_t n = 0;
+  uint8_t *dense = nullptr;
+  uint32_t *slots = nullptr;
+  uint64_t *masks = nullptr;
+  uint16_t *voxels = nullptr;
+  uint32_t *top = nullptr;
+  int64_t n_bricks = 0;
+  int64_t n_nodes = 0;
+  uint64_t bytes = 0;
+  int64_t solid = -1;
+  meso::DevScene dev() const { return {n, dense, slots, masks, voxels, top}; }
+};
+
+struct meso_event {
+  meso_context *ctx = nullptr;
+  hipEvent_t ev = nullptr;
+};
+
+struct meso_graph {
+  meso_context *ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace meso {
+// host helpers shared across translation units
+void derive_camera(const meso_camera *c, DevCamera *out);
+void derive_shading(const meso_shading *s, DevShading *out);
+hipStream_t stream_of(meso_context *ctx, meso_stream s);
+meso_status launch_zero(void *ptr, uint64_t bytes, hipStream_t stream);
+
+// kernels launched by other translation units
+meso_status launch_raymarch(meso_context *ctx, const meso_scene *scene, const DevCamera &cam,
+                            uint32_t flags, const DevShading &sh, const RayOut &out,
+                            hipStream_t stream, const RayLaunch *shape);
+RayLaunch default_ray_launch(const meso_scene *scene);
+meso_status launch_mesh_batched(meso_context *ctx, const meso_chunk_source *src, uint32_t mode,
+                                uint32_t *d_counts, uint32_t *d_prefix, uint32_t *d_quads,
+                                uint64_t capacity, uint64_t *d_total, void *d_work,
+                                hipStream_t stream, const MeshLaunch *shape);
+MeshLaunch default_mesh_launch(meso_context *ctx);
+
+// host PCG32 / SplitMix64 (same published algorithms as the generators use)
+uint64_t splitmix64(uint64_t *s);
+struct Pcg32 {
+  uint64_t state, inc;
+  void srandom(uint64_t initstate, uint64_t initseq);
+  void seed(uint64_t s);
+  uint32_t next();
+  float u01() { return (float)(next() >> 8) * (1.0f / 16777216.0f); }
+};
+}  // namespace meso

@@ -1,0 +1,444 @@
+// Device-side voxel traversal for gfx950 (Amanatides & Woo 1987 DDA with
+// hierarchical empty-space skipping).  Every float operation is written out in
+// a fixed order and the library is compiled with -ffp-contract=off, so results
+// are the IEEE binary32 results of these operations: the integer outputs
+// (voxel index, face) and the float outputs (t, shade) are reproducible
+// bit-for-bit by any scalar IEEE implementation of the same steps.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "meso_internal.h"
+
+namespace meso {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr int kFaceInside = 6;
+constexpr int kFaceMiss = 0xFF;
+
+// Face ids follow Runtimes/Shape/Cube.h:25-54: 0:+Z 1:-Z 2:+Y 3:-Y 4:+X 5:-X.
+// A ray stepping +x enters a voxel through its -X face.
+__device__ __forceinline__ int entry_face(int axis, int step) {
+  int base = axis == 2 ? 0 : (axis == 1 ? 2 : 4);
+  return base + (step > 0 ? 1 : 0);
+}
+__device__ __forceinline__ int face_axis(int f) { return f < 2 ? 2 : (f < 4 ? 1 : 0); }
+
+struct Ray {
+  float o[3], d[3], inv[3];
+  int32_t step[3];
+};
+
+__device__ __forceinline__ void ray_setup(Ray &r, const float o[3], const float d[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    r.o[a] = o[a];
+    r.d[a] = d[a];
+    if (d[a] > 0.0f) {
+      r.step[a] = 1;
+      r.inv[a] = 1.0f / d[a];
+    } else if (d[a] < 0.0f) {
+      r.step[a] = -1;
+      r.inv[a] = 1.0f / d[a];
+    } else {
+      r.step[a] = 0;
+      r.inv[a] = 0.0f;
+    }
+  }
+}
+
+__device__ __forceinline__ float plane_t(float o, float inv, float b) {
+  float q = b - o;
+  return q * inv;
+}
+__device__ __forceinline__ float next_t(const Ray &r, int a, int32_t c) {
+  if (r.step[a] > 0) return plane_t(r.o[a], r.inv[a], (float)(c + 1));
+  if (r.step[a] < 0) return plane_t(r.o[a], r.inv[a], (float)c);
+  return __builtin_inff();
+}
+// smallest t, ties to x then y
+__device__ __forceinline__ int argmin3(float t0, float t1, float t2) {
+  if (t0 <= t1 && t0 <= t2) return 0;
+  return t1 <= t2 ? 1 : 2;
+}
+
+struct WalkState {
+  int32_t cell[3];
+  float t;
+  int32_t face;
+};
+
+struct StepCounts {
+  uint32_t n_vox, n_top, n_node, n_brickslot, n_brickmask;
+};
+
+__device__ __forceinline__ int64_t lin3(int64_t x, int64_t y, int64_t z, int64_t n) {
+  return x + y * n + z * n * n;
+}
+
+// Leave the empty block (edge L) containing the current cell through its
+// nearest plane.  The other two coordinates come from the exit point, clamped
+// into the block and never moving against the ray.
+template <int L>
+__device__ __forceinline__ bool skip_block(const Ray &r, int32_t n, const int32_t bc[3],
+                                           WalkState &w, float tn[3]) {
+  float tb[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (r.step[a] > 0)
+      tb[a] = plane_t(r.o[a], r.inv[a], (float)((bc[a] + 1) * L));
+    else if (r.step[a] < 0)
+      tb[a] = plane_t(r.o[a], r.inv[a], (float)(bc[a] * L));
+    else
+      tb[a] = __builtin_inff();
+  }
+  const int ax = argmin3(tb[0], tb[1], tb[2]);
+  float t = tb[0];
+  int32_t na = 0;
+  int32_t st = 0;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+    if (a == ax) {
+      t = tb[a];
+      st = r.step[a];
+      na = st > 0 ? (bc[a] + 1) * L : bc[a] * L - 1;
+    }
+  if (na < 0 || na >= n) return false;
+#pragma unroll
+  for (int b = 0; b < 3; b++) {
+    if (b == ax) {
+      w.cell[b] = na;
+      continue;
+    }
+    float dt = r.d[b] * t;
+    float p = r.o[b] + dt;
+    int32_t c = (int32_t)__builtin_floorf(p);
+    int32_t lo = bc[b] * L, hi = lo + L - 1;
+    c = c < lo ? lo : c;
+    c = c > hi ? hi : c;
+    if (r.step[b] > 0 && c < w.cell[b]) c = w.cell[b];
+    if (r.step[b] < 0 && c > w.cell[b]) c = w.cell[b];
+    w.cell[b] = c;
+  }
+  w.t = t;
+  w.face = entry_face(ax, st);
+#pragma unroll
+  for (int a = 0; a < 3; a++) tn[a] = next_t(r, a, w.cell[a]);
+  return true;
+}
+
+// Walk until a solid voxel is entered (true), the ray leaves the scene or the
+// entry t reaches tmax (false).  KIND: 1 dense u8, 2 brick map, 3 two-level.
+template <int KIND, bool COUNT>
+__device__ __forceinline__ bool walk(const DevScene &s, const Ray &r, WalkState &w, float tmax,
+                                     uint16_t &material, StepCounts &cnt) {
+  const int32_t n = s.n;
+  float tn[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) tn[a] = next_t(r, a, w.cell[a]);
+  int32_t cur_node[3] = {-1, -1, -1};
+  int32_t cur_brick[3] = {-1, -1, -1};
+  uint32_t node = kEmpty, slot = kEmpty;
+  for (;;) {
+    if (!(w.t < tmax)) return false;
+    bool solid;
+    if constexpr (KIND == 1) {
+      if (COUNT) cnt.n_vox++;
+      uint8_t v = s.dense[lin3(w.cell[0], w.cell[1], w.cell[2], n)];
+      solid = v != 0;
+      if (solid) material = v;
+    } else {
+      if constexpr (KIND == 3) {
+        int32_t nc[3] = {w.cell[0] >> 7, w.cell[1] >> 7, w.cell[2] >> 7};
+        if (nc[0] != cur_node[0] || nc[1] != cur_node[1] || nc[2] != cur_node[2]) {
+          cur_node[0] = nc[0];
+          cur_node[1] = nc[1];
+          cur_node[2] = nc[2];
+          cur_brick[0] = cur_brick[1] = cur_brick[2] = -1;
+          if (COUNT) cnt.n_top++;
+          node = s.top[lin3(nc[0], nc[1], nc[2], n >> 7)];
+        }
+        if (node == kEmpty) {
+          if (!skip_block<128>(r, n, nc, w, tn)) return false;
+          continue;
+        }
+      }
+      int32_t bc[3] = {w.cell[0] >> 3, w.cell[1] >> 3, w.cell[2] >> 3};
+      if (bc[0] != cur_brick[0] || bc[1] != cur_brick[1] || bc[2] != cur_brick[2]) {
+        cur_brick[0] = bc[0];
+        cur_brick[1] = bc[1];
+        cur_brick[2] = bc[2];
+        if constexpr (KIND == 3) {
+          if (COUNT) cnt.n_node++;
+          slot = s.slots[(int64_t)node * 4096 + lin3(bc[0] & 15, bc[1] & 15, bc[2] & 15, 16)];
+        } else {
+          if (COUNT) cnt.n_brickslot++;
+          slot = s.slots[lin3(bc[0], bc[1], bc[2], n >> 3)];
+        }
+        if (COUNT && slot != kEmpty) cnt.n_brickmask++;
+      }
+      if (slot == kEmpty) {
+        if (!skip_block<8>(r, n, bc, w, tn)) return false;
+        continue;
+      }
+      const int32_t lx = w.cell[0] & 7, ly = w.cell[1] & 7, lz = w.cell[2] & 7;
+      const uint64_t m = s.masks[(int64_t)slot * 8 + lz];
+      solid = ((m >> (lx + 8 * ly)) & 1ull) != 0;
+      if (solid) material = s.voxels[(int64_t)slot * 512 + lx + 8 * ly + 64 * lz];
+    }
+    if (solid) return true;
+    const int ax = argmin3(tn[0], tn[1], tn[2]);
+    bool out = false;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+      if (a == ax) {
+        w.t = tn[a];
+        w.cell[a] += r.step[a];
+        out = w.cell[a] < 0 || w.cell[a] >= n;
+        w.face = entry_face(a, r.step[a]);
+        tn[a] = next_t(r, a, w.cell[a]);
+      }
+    if (out) return false;
+  }
+}
+
+__device__ __forceinline__ void primary_ray(const DevCamera &c, int32_t px, int32_t py, float o[3],
+                                            float d[3]) {
+  float a = (float)px + 0.5f;
+  a = a * c.inv_w2;
+  a = a - 1.0f;
+  a = a * c.sx;
+  float b = (float)py + 0.5f;
+  b = b * c.inv_h2;
+  b = 1.0f - b;
+  b = b * c.sy;
+  float v[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    float ra = c.right[i] * a;
+    float ub = c.up[i] * b;
+    v[i] = c.fwd[i] + ra;
+    v[i] = v[i] + ub;
+    o[i] = c.eye[i];
+  }
+  float xx = v[0] * v[0], yy = v[1] * v[1], zz = v[2] * v[2];
+  float s = xx + yy;
+  s = s + zz;
+  float len = __builtin_sqrtf(s);
+#pragma unroll
+  for (int i = 0; i < 3; i++) d[i] = v[i] / len;
+}
+
+struct Hit {
+  int64_t index;
+  int32_t face;
+  float t;
+  int32_t cell[3];
+  uint16_t material;
+};
+
+// Clip to [0,n]^3, find the entry cell and walk.
+template <int KIND, bool COUNT>
+__device__ __forceinline__ bool trace(const DevScene &s, const float o[3], const float d[3],
+                                      Hit &hit, StepCounts &cnt) {
+  Ray r;
+  ray_setup(r, o, d);
+  const int32_t n = s.n;
+  const float fn = (float)n;
+  hit.index = -1;
+  hit.face = kFaceMiss;
+  hit.t = __builtin_inff();
+  hit.material = 0;
+  float t_enter = 0.0f, t_exit = __builtin_inff();
+  int ea = -1;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (r.step[a] == 0) {
+      if (o[a] < 0.0f || o[a] >= fn) return false;
+      continue;
+    }
+    float lo = plane_t(r.o[a], r.inv[a], r.step[a] > 0 ? 0.0f : fn);
+    float hi = plane_t(r.o[a], r.inv[a], r.step[a] > 0 ? fn : 0.0f);
+    if (lo > t_enter) {
+      t_enter = lo;
+      ea = a;
+    }
+    if (hi < t_exit) t_exit = hi;
+  }
+  if (t_enter > t_exit) return false;
+  WalkState w;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (a == ea) {
+      w.cell[a] = r.step[a] > 0 ? 0 : n - 1;
+    } else {
+      float dt = d[a] * t_enter;
+      float p = o[a] + dt;
+      int32_t c = (int32_t)__builtin_floorf(p);
+      c = c < 0 ? 0 : c;
+      c = c > n - 1 ? n - 1 : c;
+      w.cell[a] = c;
+    }
+  }
+  w.t = t_enter;
+  int est = 0;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+    if (a == ea) est = r.step[a];
+  w.face = ea >= 0 ? entry_face(ea, est) : kFaceInside;
+  uint16_t m = 0;
+  if (!walk<KIND, COUNT>(s, r, w, __builtin_inff(), m, cnt)) return false;
+  hit.index = lin3(w.cell[0], w.cell[1], w.cell[2], n);
+  hit.face = w.face;
+  hit.t = w.t;
+  hit.cell[0] = w.cell[0];
+  hit.cell[1] = w.cell[1];
+  hit.cell[2] = w.cell[2];
+  hit.material = m;
+  return true;
+}
+
+template <int KIND, bool COUNT>
+__device__ __forceinline__ bool occluded(const DevScene &s, const float o[3], const float d[3],
+                                         const int32_t start[3], float tmax, StepCounts &cnt) {
+  const int32_t n = s.n;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+    if (start[a] < 0 || start[a] >= n) return false;
+  Ray r;
+  ray_setup(r, o, d);
+  WalkState w;
+  w.cell[0] = start[0];
+  w.cell[1] = start[1];
+  w.cell[2] = start[2];
+  w.t = 0.0f;
+  w.face = kFaceInside;
+  uint16_t m;
+  return walk<KIND, COUNT>(s, r, w, tmax, m, cnt);
+}
+
+// lowbias32 integer finaliser (C. Wellons, hash-prospector)
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// Cosine-weighted hemisphere direction about the face normal (Malley's
+// method): a point uniform in the unit disk, found by rejection from
+// counter-based hashes, lifted by z = sqrt(1 - r^2).
+__device__ __forceinline__ void ao_direction(uint32_t seed, uint32_t pixel, uint32_t sample,
+                                             int face, float dir[3]) {
+  float a = 0.0f, b = 0.0f, r2 = 0.0f;
+  const uint32_t base = hash32(hash32(seed) ^ pixel);
+  bool ok = false;
+  for (uint32_t k = 0; k < 16u; k++) {
+    uint32_t h = hash32(base ^ (sample * 16u + k));
+    a = (float)(h & 0xFFFFu) * (1.0f / 32768.0f);
+    a = a - 1.0f;
+    b = (float)(h >> 16) * (1.0f / 32768.0f);
+    b = b - 1.0f;
+    float aa = a * a, bb = b * b;
+    r2 = aa + bb;
+    if (r2 < 1.0f) {
+      ok = true;
+      break;
+    }
+  }
+  if (!ok) {
+    a = 0.0f;
+    b = 0.0f;
+    r2 = 0.0f;
+  }
+  float z = __builtin_sqrtf(1.0f - r2);
+  const int na = face_axis(face);
+  const bool pos = (face & 1) == 0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int ua = na == 0 ? 1 : 0;
+    const int va = na == 2 ? 1 : 2;
+    if (k == na) dir[k] = pos ? z : -z;
+    else if (k == ua) dir[k] = a;
+    else if (k == va) dir[k] = b;
+  }
+}
+
+struct PixelResult {
+  int64_t index;
+  int32_t face;
+  float t;
+  float shade;
+};
+
+// Primary ray of pixel (px,py), optional shadow ray toward the light and AO
+// rays, shaded as (0.25 + 0.75 * lambert * visible) * (0.5 + 0.5 * unoccluded).
+template <int KIND, bool COUNT>
+__device__ __forceinline__ PixelResult shade_pixel(const DevScene &s, const DevCamera &cam,
+                                                   uint32_t flags, const DevShading &sh,
+                                                   int32_t px, int32_t py, StepCounts &cnt,
+                                                   uint32_t &n_shadow, uint32_t &n_ao) {
+  float o[3], d[3];
+  primary_ray(cam, px, py, o, d);
+  Hit h;
+  PixelResult res;
+  const bool hit = trace<KIND, COUNT>(s, o, d, h, cnt);
+  float shade = 0.0f;
+  if (hit) {
+    if (h.face < 6) {
+      const int na = face_axis(h.face);
+      const int sg = (h.face & 1) ? -1 : 1;
+      float lightc = sh.light[0];
+      int32_t start[3] = {h.cell[0], h.cell[1], h.cell[2]};
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+        if (k == na) {
+          lightc = sh.light[k];
+          start[k] += sg;
+        }
+      float lambert = sg > 0 ? lightc : -lightc;
+      if (lambert < 0.0f) lambert = 0.0f;
+      float p[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        float dt = d[i] * h.t;
+        p[i] = o[i] + dt;
+      }
+      float vis = 1.0f;
+      if ((flags & MESO_RAY_SHADOW) && lambert > 0.0f) {
+        n_shadow++;
+        if (occluded<KIND, COUNT>(s, p, sh.light, start, __builtin_inff(), cnt)) vis = 0.0f;
+      }
+      float ao = 1.0f;
+      if (flags & MESO_RAY_AO) {
+        int32_t unocc = 0;
+        const uint32_t pix = (uint32_t)((int64_t)py * cam.width + px);
+        for (int32_t k = 0; k < sh.ao_samples; k++) {
+          float dir[3];
+          ao_direction(sh.ao_seed, pix, (uint32_t)k, h.face, dir);
+          n_ao++;
+          if (!occluded<KIND, COUNT>(s, p, dir, start, sh.ao_radius, cnt)) unocc++;
+        }
+        float frac = (float)unocc / (float)sh.ao_samples;
+        frac = 0.5f * frac;
+        ao = 0.5f + frac;
+      }
+      float l = lambert * vis;
+      l = 0.75f * l;
+      shade = 0.25f + l;
+      shade = shade * ao;
+    } else {
+      shade = 0.25f;
+    }
+  }
+  res.index = h.index;
+  res.face = h.face;
+  res.t = h.t;
+  res.shade = shade;
+  return res;
+}
+
+}  // namespace meso

@@ -1,0 +1,335 @@
+"""Python harness over the C-ABI: device buffers, scenes, raymarch, meshing,
+serving.  Every call goes through libmeso_hip.so; there is no CPU fallback."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import byref, check
+
+
+class Context:
+    def __init__(self, device=0):
+        self.lib = L.load()
+        h = C.c_void_p()
+        check(self.lib.meso_context_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.meso_context_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def device_info(self):
+        d = L.DeviceInfo()
+        d.size = C.sizeof(d)
+        check(self.lib.meso_device_info_get(self.h, byref(d)))
+        return d
+
+    def sync(self, stream=None):
+        check(self.lib.meso_stream_sync(self.h, stream))
+
+    # ---------------- buffers
+    def buffer(self, nbytes, init=None):
+        return Buffer(self, nbytes, init)
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        return Buffer(self, max(a.nbytes, 1), a if a.nbytes else None)
+
+    def zeros(self, nbytes, stream=None):
+        b = Buffer(self, max(int(nbytes), 16))
+        check(self.lib.meso_buffer_fill_zero(b.h, stream))
+        return b
+
+    # ---------------- scenes
+    def scene_generate(self, generator, layout, n, seed, n_spheres=8):
+        d = L.GenerateDesc(C.sizeof(L.GenerateDesc), generator, layout, n, seed, n_spheres)
+        h = C.c_void_p()
+        check(self.lib.meso_scene_generate(self.h, byref(d), C.byref(h)))
+        return Scene(self, h)
+
+    def scene_dense(self, grid_u8):
+        g = np.ascontiguousarray(grid_u8, dtype=np.uint8)
+        n = g.shape[0]
+        h = C.c_void_p()
+        check(self.lib.meso_scene_create_dense_u8(self.h, n, g.ctypes.data, C.byref(h)))
+        return Scene(self, h)
+
+    def scene_brickmap(self, n, slots, masks, voxels):
+        s = np.ascontiguousarray(slots, np.uint32)
+        m = np.ascontiguousarray(masks, np.uint64)
+        v = np.ascontiguousarray(voxels, np.uint16)
+        h = C.c_void_p()
+        check(self.lib.meso_scene_create_brickmap_u16(self.h, n, s.ctypes.data, len(m),
+                                                      m.ctypes.data if len(m) else None,
+                                                      v.ctypes.data if len(v) else None,
+                                                      C.byref(h)))
+        return Scene(self, h)
+
+    def scene_two_level(self, n, top, node_slots, masks, voxels):
+        t = np.ascontiguousarray(top, np.uint32)
+        ns = np.ascontiguousarray(node_slots, np.uint32)
+        m = np.ascontiguousarray(masks, np.uint64)
+        v = np.ascontiguousarray(voxels, np.uint16)
+        h = C.c_void_p()
+        check(self.lib.meso_scene_create_two_level_u16(
+            self.h, n, t.ctypes.data, len(ns) // 4096, ns.ctypes.data if len(ns) else None,
+            len(m), m.ctypes.data if len(m) else None, v.ctypes.data if len(v) else None,
+            C.byref(h)))
+        return Scene(self, h)
+
+
+class Buffer:
+    def __init__(self, ctx, nbytes, init=None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.nbytes = int(nbytes)
+        h = C.c_void_p()
+        src = None
+        if init is not None:
+            init = np.ascontiguousarray(init)
+            assert init.nbytes == self.nbytes
+            src = init.ctypes.data
+        check(self.lib.meso_buffer_create(ctx.h, self.nbytes, src, C.byref(h)))
+        self.h = h
+        self.ptr = self.lib.meso_buffer_device_address(h, 0)
+        if not self.ptr:
+            raise L.MesoError(L.ERR_RUNTIME, "no device address")
+
+    def download(self, dtype, count=None, offset=0):
+        dt = np.dtype(dtype)
+        if count is None:
+            count = (self.nbytes - offset) // dt.itemsize
+        out = np.empty(count, dt)
+        if count:
+            check(self.lib.meso_buffer_download(self.h, out.ctypes.data, out.nbytes, offset))
+        return out
+
+    def upload(self, arr, offset=0):
+        a = np.ascontiguousarray(arr)
+        check(self.lib.meso_buffer_upload(self.h, a.ctypes.data, a.nbytes, offset))
+
+    def zero(self, stream=None):
+        check(self.lib.meso_buffer_fill_zero(self.h, stream))
+
+    def free(self):
+        if self.h:
+            self.lib.meso_buffer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Scene:
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = h
+        self.lib = ctx.lib
+
+    def info(self):
+        i = L.SceneInfo()
+        i.size = C.sizeof(i)
+        check(self.lib.meso_scene_get_info(self.h, byref(i)))
+        return i
+
+    def download(self, part, dtype, count):
+        out = np.empty(count, dtype)
+        if count:
+            check(self.lib.meso_scene_download(self.h, part, out.ctypes.data, out.nbytes))
+        return out
+
+    def arrays(self):
+        """Download the scene's arrays (for parity checks)."""
+        i = self.info()
+        n = i.n
+        if i.kind == L.SCENE_DENSE_U8:
+            return {"dense": self.download(L.PART_DENSE, np.uint8, n ** 3)}
+        out = {"masks": self.download(L.PART_MASKS, np.uint64, i.n_bricks * 8).reshape(-1, 8),
+               "voxels": self.download(L.PART_VOXELS, np.uint16, i.n_bricks * 512).reshape(-1, 512)}
+        if i.kind == L.SCENE_BRICKMAP_U16:
+            out["slots"] = self.download(L.PART_SLOTS, np.uint32, (n // 8) ** 3)
+        else:
+            out["top"] = self.download(L.PART_TOP, np.uint32, (n // 128) ** 3)
+            out["slots"] = self.download(L.PART_SLOTS, np.uint32, i.n_nodes * 4096)
+        return out
+
+    def destroy(self):
+        if self.h:
+            self.lib.meso_scene_destroy(self.h)
+            self.h = None
+
+
+def camera(eye, target, up, vfov_deg, width, height):
+    c = L.Camera()
+    c.size = C.sizeof(c)
+    c.eye[:] = [float(v) for v in eye]
+    c.target[:] = [float(v) for v in target]
+    c.up[:] = [float(v) for v in up]
+    c.vfov_deg = vfov_deg
+    c.width, c.height = width, height
+    return c
+
+
+def shading(light=(0.4, 1.0, 0.3), ao_seed=0, ao_samples=4, ao_radius=16.0):
+    s = L.Shading()
+    s.size = C.sizeof(s)
+    s.light[:] = [float(v) for v in light]
+    s.ao_seed, s.ao_samples, s.ao_radius = ao_seed, ao_samples, ao_radius
+    return s
+
+
+class RayTargets:
+    """Device output buffers for one image."""
+
+    def __init__(self, ctx, width, height, index_bytes=4, counters=False):
+        self.n = width * height
+        self.index_bytes = index_bytes
+        self.index = ctx.buffer(self.n * index_bytes)
+        self.face = ctx.buffer(self.n)
+        self.t = ctx.buffer(self.n * 4)
+        self.shade = ctx.buffer(self.n * 4)
+        self.counters = ctx.zeros(8 * len(L.CNT_NAMES)) if counters else None
+        o = L.RayOutputs()
+        o.size = C.sizeof(o)
+        o.hit_index, o.index_bytes = self.index.ptr, index_bytes
+        o.face, o.t, o.shade = self.face.ptr, self.t.ptr, self.shade.ptr
+        o.counters = self.counters.ptr if counters else None
+        self.out = o
+
+    def results(self):
+        idx = self.index.download(np.int64 if self.index_bytes == 8 else np.int32)
+        return (idx.astype(np.int64), self.face.download(np.uint8), self.t.download(np.float32),
+                self.shade.download(np.float32))
+
+    def counts(self):
+        c = self.counters.download(np.uint64)
+        return dict(zip(L.CNT_NAMES, (int(v) for v in c)))
+
+
+def raymarch(ctx, scene, cam, flags, sh, targets, stream=None):
+    check(ctx.lib.meso_raymarch(ctx.h, scene.h, byref(cam), flags, byref(sh) if sh else None,
+                                byref(targets.out), stream))
+
+
+def chunk_source(buf, elem_bytes, n_chunks, layout=L.CHUNKS_PACKED, grid_n=0):
+    s = L.ChunkSource()
+    s.size = C.sizeof(s)
+    s.voxels = buf.ptr
+    s.elem_bytes, s.layout, s.grid_n, s.n_chunks = elem_bytes, layout, grid_n, n_chunks
+    return s
+
+
+class MeshTargets:
+    def __init__(self, ctx, n_chunks, capacity):
+        self.n_chunks = n_chunks
+        self.capacity = int(capacity)
+        self.counts = ctx.buffer(4 * n_chunks)
+        self.prefix = ctx.buffer(4 * n_chunks)
+        self.quads = ctx.buffer(4 * max(self.capacity, 1))
+        self.total = ctx.buffer(16)
+        self.work = ctx.buffer(ctx.lib.meso_mesh_workspace_bytes(n_chunks))
+
+    def results(self):
+        counts = self.counts.download(np.uint32)
+        prefix = self.prefix.download(np.uint32)
+        total = int(self.total.download(np.uint64, 1)[0])
+        quads = self.quads.download(np.uint32, min(total, self.capacity))
+        return counts, prefix, total, quads
+
+
+def mesh_chunks(ctx, src, mode, mt, stream=None):
+    check(ctx.lib.meso_mesh_chunks(ctx.h, byref(src), mode, mt.counts.ptr, mt.prefix.ptr,
+                                   mt.quads.ptr, mt.capacity, mt.total.ptr, mt.work.ptr, stream))
+
+
+def mesh_chunk(ctx, vox_ptr, elem_bytes, mode, count_ptr, quads_ptr, capacity, stream=None):
+    check(ctx.lib.meso_mesh_chunk(ctx.h, vox_ptr, elem_bytes, mode, count_ptr, quads_ptr,
+                                  capacity, stream))
+
+
+class Event:
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(ctx.lib.meso_event_create(ctx.h, C.byref(h)))
+        self.h = h
+
+    def record(self, stream=None):
+        check(self.ctx.lib.meso_event_record(self.h, stream))
+
+    def sync(self):
+        check(self.ctx.lib.meso_event_sync(self.h))
+
+    def elapsed_ms(self, end):
+        ms = C.c_float()
+        check(self.ctx.lib.meso_event_elapsed_ms(self.h, end.h, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            self.ctx.lib.meso_event_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Server:
+    def __init__(self, ctx, budget_bytes, max_pending=256, max_inflight=2,
+                 rays_per_launch=1 << 20):
+        self.ctx = ctx
+        cfg = L.ServeConfig(C.sizeof(L.ServeConfig), budget_bytes, max_pending, max_inflight,
+                            rays_per_launch, 16)
+        h = C.c_void_p()
+        check(ctx.lib.meso_serve_start(ctx.h, byref(cfg), C.byref(h)))
+        self.h = h
+
+    def submit(self, scene, cam, flags, sh):
+        t = C.c_uint64()
+        st = self.ctx.lib.meso_serve_submit(self.h, scene.h, byref(cam), flags,
+                                            byref(sh) if sh else None, C.byref(t))
+        if st == L.ERR_BUSY:
+            return None
+        check(st)
+        return t.value
+
+    def poll(self, ticket):
+        d = C.c_int32()
+        check(self.ctx.lib.meso_serve_poll(self.h, ticket, C.byref(d)))
+        return d.value
+
+    def wait(self, ticket, timeout_us=-1):
+        lat = C.c_double()
+        check(self.ctx.lib.meso_serve_wait(self.h, ticket, timeout_us, C.byref(lat)))
+        return lat.value
+
+    def take(self, ticket, width, height, index_bytes=4):
+        n = width * height
+        idx = np.empty(n, np.int64 if index_bytes == 8 else np.int32)
+        face = np.empty(n, np.uint8)
+        t = np.empty(n, np.float32)
+        shade = np.empty(n, np.float32)
+        o = L.HostOutputs(C.sizeof(L.HostOutputs), index_bytes, idx.ctypes.data,
+                          face.ctypes.data, t.ctypes.data, shade.ctypes.data)
+        check(self.ctx.lib.meso_serve_take(self.h, ticket, byref(o)))
+        return idx.astype(np.int64), face, t, shade
+
+    def stats(self):
+        s = L.ServeStats()
+        s.size = C.sizeof(s)
+        check(self.ctx.lib.meso_serve_stats_get(self.h, byref(s)))
+        return {k: getattr(s, k) for k, _ in L.ServeStats._fields_ if k != "size"}
+
+    def stop(self):
+        if self.h:
+            check(self.ctx.lib.meso_serve_stop(self.h))
+            self.h = None

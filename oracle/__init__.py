@@ -1,0 +1,341 @@
+"""ctypes binding of the CPU oracle (oracle/meso_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.  The product library
+(mesoengine_amd) never imports this package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+EMPTY_SLOT = 0xFFFFFFFF
+FACE_MISS = 0xFF
+FACE_INSIDE = 6
+SCENE_DENSE_U8, SCENE_BRICKMAP_U16, SCENE_TWO_LEVEL_U16 = 1, 2, 3
+GEN_HEIGHTFIELD, GEN_TERRAIN_CAVES, GEN_CAVES = 1, 2, 3
+RAY_SHADOW, RAY_AO = 1, 2
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+class Camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("fwd", C.c_float * 3), ("right", C.c_float * 3),
+                ("up", C.c_float * 3), ("sx", C.c_float), ("sy", C.c_float),
+                ("inv_w2", C.c_float), ("inv_h2", C.c_float),
+                ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class Scene(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("n", C.c_int32), ("dense", C.c_void_p),
+                ("slots", C.c_void_p), ("masks", C.c_void_p), ("voxels", C.c_void_p),
+                ("top", C.c_void_p)]
+
+
+class Counters(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in ("n_vox", "n_top", "n_node", "n_brickslot",
+                                          "n_brickmask", "hits", "rays", "shadow_rays",
+                                          "ao_rays")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Hit(C.Structure):
+    _fields_ = [("index", C.c_int64), ("face", C.c_int32), ("t", C.c_float),
+                ("cell", C.c_int32 * 3), ("material", C.c_uint16)]
+
+
+class Shading(C.Structure):
+    _fields_ = [("light", C.c_float * 3), ("ao_seed", C.c_uint32), ("ao_samples", C.c_int32),
+                ("ao_radius", C.c_float)]
+
+
+class ChunkParams(C.Structure):
+    _fields_ = [("fill", C.c_float), ("noise_seed", C.c_uint32), ("n_spheres", C.c_int32),
+                ("sph", (C.c_int32 * 4) * 4)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        P = C.c_void_p
+        sig = {
+            "mo_quad_pack": (C.c_uint32, [C.c_int] * 6),
+            "mo_splitmix64": (C.c_uint64, [P]),
+            "mo_pcg32_srandom": (None, [P, C.c_uint64, C.c_uint64]),
+            "mo_pcg32_next": (C.c_uint32, [P]),
+            "mo_pcg32_seed": (None, [P, C.c_uint64]),
+            "mo_hash32": (C.c_uint32, [C.c_uint32]),
+            "mo_lattice_hash3": (C.c_uint32, [C.c_int32] * 3 + [C.c_uint32]),
+            "mo_value_noise3": (C.c_float, [C.c_float] * 3 + [C.c_uint32]),
+            "mo_value_noise2": (C.c_float, [C.c_float] * 2 + [C.c_uint32]),
+            "mo_fbm2": (C.c_float, [C.c_float, C.c_float, C.c_int, C.c_float, C.c_uint32]),
+            "mo_fbm3": (C.c_float, [C.c_float] * 3 + [C.c_int, C.c_float, C.c_uint32]),
+            "mo_sphere_params": (None, [C.c_uint64, C.c_int, C.c_int, P]),
+            "mo_gen_spheres_dense": (None, [P, C.c_int, C.c_int, P]),
+            "mo_height": (C.c_int32, [C.c_int32] * 3 + [C.c_uint32]),
+            "mo_scene_voxel": (C.c_uint16, [C.c_int] + [C.c_int32] * 4 + [C.c_uint32]),
+            "mo_build_brickmap": (C.c_int64, [C.c_int, C.c_int32, C.c_uint32, P, P, P, C.c_int64]),
+            "mo_brickmap_from_dense": (C.c_int64, [P, C.c_int32, P, P, P, C.c_int64]),
+            "mo_gen_brick": (C.c_int32, [C.c_int, C.c_int32, C.c_uint32] + [C.c_int32] * 3 + [P, P]),
+            "mo_chunk_params_gen": (None, [C.c_uint64, C.c_int32, P]),
+            "mo_gen_chunk": (None, [P, P]),
+            "mo_camera_derive": (None, [P, P, P, C.c_float, C.c_int32, C.c_int32, P]),
+            "mo_primary_ray": (None, [P, C.c_int32, C.c_int32, P, P]),
+            "mo_trace": (C.c_int, [P, P, P, P, P]),
+            "mo_occluded": (C.c_int, [P, P, P, P, C.c_float, P]),
+            "mo_raymarch": (None, [P, P, C.c_uint32, P, C.c_int32, C.c_int32, P, P, P, P, P]),
+            "mo_raymarch_mt": (None, [P, P, C.c_uint32, P, C.c_int32, C.c_int32, C.c_int32,
+                                      P, P, P, P, P]),
+            "mo_ao_direction": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, P]),
+            "mo_mesh_chunk": (C.c_uint32, [P, C.c_int, C.c_int, P, C.c_uint32]),
+            "mo_mesh_chunks_mt": (C.c_uint64, [P, C.c_int, C.c_int32, C.c_int, C.c_int32, P]),
+            "mo_ref_erode_mips": (None, [P, C.c_int32, C.c_int32, P]),
+            "mo_ref_generate_sphere": (C.c_int32, [C.c_int32] * 3 + [C.c_float, C.c_int32, P]),
+            "mo_ref_test_generator": (C.c_int32, [C.c_int32] * 3 + [C.c_float, C.c_int32, P]),
+            "mo_ref_octant_id": (C.c_int32, [C.c_float] * 3),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------- RNG ----------------
+class PCG32(C.Structure):
+    _fields_ = [("state", C.c_uint64), ("inc", C.c_uint64)]
+
+
+def pcg32_srandom(initstate, initseq):
+    r = PCG32()
+    lib().mo_pcg32_srandom(C.byref(r), initstate, initseq)
+    return r
+
+
+def pcg32_next(r):
+    return lib().mo_pcg32_next(C.byref(r))
+
+
+def splitmix64_seq(seed, n):
+    s = C.c_uint64(seed)
+    return [lib().mo_splitmix64(C.byref(s)) for _ in range(n)]
+
+
+# ---------------- generators ----------------
+def sphere_params(seed, n, n_spheres):
+    p = np.zeros(4 * n_spheres, np.int32)
+    lib().mo_sphere_params(seed, n, n_spheres, _p(p))
+    return p.reshape(n_spheres, 4)
+
+
+def spheres_dense(seed, n=128, n_spheres=8):
+    p = sphere_params(seed, n, n_spheres)
+    g = np.zeros(n ** 3, np.uint8)
+    lib().mo_gen_spheres_dense(_p(np.ascontiguousarray(p.ravel())), n_spheres, n, _p(g))
+    return g.reshape(n, n, n)  # [z][y][x]
+
+
+def build_brickmap(kind, n, seed):
+    nb = n // 8
+    slots = np.zeros(nb ** 3, np.uint32)
+    cnt = lib().mo_build_brickmap(kind, n, seed, _p(slots), None, None, 0)
+    masks = np.zeros((max(cnt, 1), 8), np.uint64)
+    vox = np.zeros((max(cnt, 1), 512), np.uint16)
+    cnt2 = lib().mo_build_brickmap(kind, n, seed, _p(slots), _p(masks), _p(vox), cnt)
+    assert cnt2 == cnt
+    return slots, masks[:cnt], vox[:cnt]
+
+
+def brickmap_from_dense(grid_u16):
+    n = grid_u16.shape[0]
+    nb = n // 8
+    g = np.ascontiguousarray(grid_u16, dtype=np.uint16)
+    slots = np.zeros(nb ** 3, np.uint32)
+    cnt = lib().mo_brickmap_from_dense(_p(g), n, _p(slots), None, None, 0)
+    masks = np.zeros((max(cnt, 1), 8), np.uint64)
+    vox = np.zeros((max(cnt, 1), 512), np.uint16)
+    lib().mo_brickmap_from_dense(_p(g), n, _p(slots), _p(masks), _p(vox), cnt)
+    return slots, masks[:cnt], vox[:cnt]
+
+
+def gen_brick(kind, n, seed, bx, by, bz):
+    vox = np.zeros(512, np.uint16)
+    mask = np.zeros(8, np.uint64)
+    s = lib().mo_gen_brick(kind, n, seed, bx, by, bz, _p(vox), _p(mask))
+    return s, vox, mask
+
+
+def chunk_params(seed, n_chunks):
+    arr = (ChunkParams * n_chunks)()
+    lib().mo_chunk_params_gen(seed, n_chunks, C.cast(arr, C.c_void_p))
+    return arr
+
+
+def gen_chunks(seed, n_chunks):
+    params = chunk_params(seed, n_chunks)
+    out = np.zeros((n_chunks, 32768), np.uint16)
+    for c in range(n_chunks):
+        lib().mo_gen_chunk(C.cast(C.byref(params[c]), C.c_void_p), _p(out[c]))
+    return out, params
+
+
+# ---------------- camera / traversal ----------------
+def camera(eye, target, up, vfov_deg, width, height):
+    cam = Camera()
+    f3 = C.c_float * 3
+    lib().mo_camera_derive(C.cast(f3(*eye), C.c_void_p), C.cast(f3(*target), C.c_void_p),
+                           C.cast(f3(*up), C.c_void_p), vfov_deg, width, height,
+                           C.cast(C.byref(cam), C.c_void_p))
+    return cam
+
+
+def primary_ray(cam, px, py):
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    lib().mo_primary_ray(C.cast(C.byref(cam), C.c_void_p), px, py, _p(o), _p(d))
+    return o, d
+
+
+class OracleScene:
+    """Keeps the numpy arrays alive behind an mo_scene struct."""
+
+    def __init__(self, kind, n, dense=None, slots=None, masks=None, voxels=None, top=None):
+        self.arrays = [a for a in (dense, slots, masks, voxels, top) if a is not None]
+        self.s = Scene(kind, n, _p(dense), _p(slots), _p(masks), _p(voxels), _p(top))
+        self.kind, self.n = kind, n
+
+    @property
+    def ptr(self):
+        return C.cast(C.byref(self.s), C.c_void_p)
+
+
+def dense_scene(grid_u8):
+    g = np.ascontiguousarray(grid_u8, dtype=np.uint8).ravel()
+    return OracleScene(SCENE_DENSE_U8, int(round(g.size ** (1 / 3))), dense=g)
+
+
+def brickmap_scene(n, slots, masks, voxels):
+    return OracleScene(SCENE_BRICKMAP_U16, n, slots=np.ascontiguousarray(slots),
+                       masks=np.ascontiguousarray(masks), voxels=np.ascontiguousarray(voxels))
+
+
+def two_level_scene(n, top, node_slots, masks, voxels):
+    return OracleScene(SCENE_TWO_LEVEL_U16, n, top=np.ascontiguousarray(top),
+                       slots=np.ascontiguousarray(node_slots), masks=np.ascontiguousarray(masks),
+                       voxels=np.ascontiguousarray(voxels))
+
+
+def trace(scene, o, d):
+    h = Hit()
+    cnt = Counters()
+    o = np.asarray(o, np.float32)
+    d = np.asarray(d, np.float32)
+    lib().mo_trace(scene.ptr, _p(o), _p(d), C.cast(C.byref(h), C.c_void_p),
+                   C.cast(C.byref(cnt), C.c_void_p))
+    return h, cnt
+
+
+def occluded(scene, o, d, start, tmax=float("inf")):
+    cnt = Counters()
+    o = np.asarray(o, np.float32)
+    d = np.asarray(d, np.float32)
+    st = np.asarray(start, np.int32)
+    r = lib().mo_occluded(scene.ptr, _p(o), _p(d), _p(st), tmax, C.cast(C.byref(cnt), C.c_void_p))
+    return bool(r), cnt
+
+
+def shading(light=(0.4, 1.0, 0.3), ao_seed=0, ao_samples=4, ao_radius=16.0):
+    lx, ly, lz = (np.float32(v) for v in light)
+    ln = np.sqrt(np.float32(np.float32(lx * lx) + np.float32(ly * ly)) + np.float32(lz * lz),
+                 dtype=np.float32)
+    L = (C.c_float * 3)(float(lx / ln), float(ly / ln), float(lz / ln))
+    return Shading(L, ao_seed, ao_samples, ao_radius)
+
+
+def raymarch(scene, cam, flags, sh, row0=0, row1=None, threads=1):
+    W, H = cam.width, cam.height
+    row1 = H if row1 is None else row1
+    n = W * H
+    idx = np.full(n, -1, np.int64)
+    face = np.full(n, FACE_MISS, np.uint8)
+    t = np.full(n, np.inf, np.float32)
+    shade = np.zeros(n, np.float32)
+    cnt = Counters()
+    V = C.c_void_p
+    if threads > 1:
+        lib().mo_raymarch_mt(scene.ptr, C.cast(C.byref(cam), V), flags, C.cast(C.byref(sh), V),
+                             row0, row1, threads, _p(idx), _p(face), _p(t), _p(shade),
+                             C.cast(C.byref(cnt), V))
+    else:
+        lib().mo_raymarch(scene.ptr, C.cast(C.byref(cam), V), flags, C.cast(C.byref(sh), V),
+                          row0, row1, _p(idx), _p(face), _p(t), _p(shade), C.cast(C.byref(cnt), V))
+    return idx, face, t, shade, cnt
+
+
+def ao_direction(seed, pixel, sample, face):
+    d = np.zeros(3, np.float32)
+    lib().mo_ao_direction(seed, pixel, sample, face, _p(d))
+    return d
+
+
+# ---------------- meshing ----------------
+def mesh_chunk(vox, greedy):
+    v = np.ascontiguousarray(vox)
+    es = v.dtype.itemsize
+    cnt = lib().mo_mesh_chunk(_p(v), es, int(greedy), None, 0)
+    q = np.zeros(max(cnt, 1), np.uint32)
+    lib().mo_mesh_chunk(_p(v), es, int(greedy), _p(q), cnt)
+    return q[:cnt]
+
+
+def mesh_chunks_counts(vox, greedy, threads=1):
+    v = np.ascontiguousarray(vox)
+    n_chunks = v.shape[0]
+    counts = np.zeros(n_chunks, np.uint32)
+    total = lib().mo_mesh_chunks_mt(_p(v), v.dtype.itemsize, n_chunks, int(greedy), threads,
+                                    _p(counts))
+    return counts, total
+
+
+def unpack_quads(q):
+    q = np.asarray(q, np.uint32)
+    return {"x": q & 31, "y": (q >> 5) & 31, "z": (q >> 10) & 31, "w": ((q >> 15) & 31) + 1,
+            "h": ((q >> 20) & 31) + 1, "face": (q >> 25) & 7, "pad": q >> 28}
+
+
+# ---------------- reference restatements ----------------
+def ref_erode_mips(occ, depth=4):
+    occ = np.ascontiguousarray(occ, dtype=np.uint8)
+    res = occ.shape[0]
+    mips = np.zeros((depth,) + occ.shape, np.uint8)
+    lib().mo_ref_erode_mips(_p(occ), res, depth, _p(mips))
+    return mips
+
+
+def ref_generate_sphere(cx, cy, cz, block_size=1.0, res=16):
+    cells = np.zeros((res ** 3, 3), np.uint8)
+    k = lib().mo_ref_generate_sphere(cx, cy, cz, block_size, res, _p(cells))
+    return cells[:k]
+
+
+def ref_test_generator(cx, cy, cz, block_size=1.0, res=16):
+    cells = np.zeros((res ** 3, 3), np.uint8)
+    k = lib().mo_ref_test_generator(cx, cy, cz, block_size, res, _p(cells))
+    return cells[:k]

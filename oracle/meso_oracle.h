@@ -1,0 +1,219 @@
+/*
+ * meso_oracle.h -- CPU oracle for the MesoEngine voxel hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This scalar C restatement is the checker that
+ * the HIP library (mesoengine_amd/csrc) is compared against.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product never links, loads or calls it.
+ *
+ * The reference (yuchengzhong/MesoEngine) has no DDA raymarcher and no
+ * face-cull/greedy mesher.  The traversal follows Amanatides & Woo (1987),
+ * "A Fast Voxel Traversal Algorithm for Ray Tracing".  The greedy merge
+ * follows M. Lysenko (2012), "Meshing in a Minecraft game".  Both are
+ * restated here over the reference's data conventions:
+ *   - linear index x + y*Rx + z*Rx*Ry, x fastest
+ *     (Runtimes/Helper/VoxelMathHelper.h:73-76)
+ *   - [0,n) bound tests (VoxelMathHelper.h:98-104); outside the grid or
+ *     chunk counts as empty (BinaryOccupancyVolume.h:43-51)
+ *   - face enumeration 0:+Z 1:-Z 2:+Y 3:-Y 4:+X 5:-X, the vertex-group order
+ *     of Runtimes/Shape/Cube.h:25-54
+ *   - brick edge 8 (Runtimes/Voxel/VoxelSceneConfig.h:22), a top node of 16^3
+ *     bricks (VoxelSceneConfig.h:24), and an 8^3 bit occupancy mask of 64 B
+ *     per brick (Runtimes/Voxel/VoxelStructure.h:30-39)
+ *   - packed little-endian u32 records (Runtimes/Voxel/Block/Block.h:21-27,
+ *     unpacked in Samples/SimpleVoxel.cpp:79-85)
+ *
+ * Pins (each function below is checked by tests/ against something other
+ * than itself):
+ *   PCG32 / SplitMix64: published known-answer outputs.
+ *   Generators: numpy re-evaluation at integer/fixed-point level (spheres);
+ *     closed-form lattice counts (reference sphere); invariants.
+ *   DDA: brute-force FP64 slab intersection against every solid voxel on
+ *     tiny grids; closed forms (single voxel, axis-aligned box, axis-parallel
+ *     rays); FP64 entry-plane t of the chosen voxel.
+ *   Meshers: brute-force face enumeration; closed forms (a x b x c box);
+ *     greedy tiling invariants; erosion-cull cross-check.
+ *   Reference TestGenerator (sin hash): parity unpinned (CPU timing only).
+ *
+ * Build: cc -O2 -ffp-contract=off (no -ffast-math), so every float result is
+ * the IEEE-754 single-precision result of the operations as written.
+ */
+#ifndef MESO_ORACLE_H
+#define MESO_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- conventions ---------------- */
+#define MO_FACE_PZ 0
+#define MO_FACE_NZ 1
+#define MO_FACE_PY 2
+#define MO_FACE_NY 3
+#define MO_FACE_PX 4
+#define MO_FACE_NX 5
+#define MO_FACE_INSIDE 6   /* ray origin inside a solid voxel */
+#define MO_FACE_MISS 0xFF
+#define MO_EMPTY_SLOT 0xFFFFFFFFu
+#define MO_BRICK 8
+#define MO_NODE_BRICKS 16  /* top node = 16^3 bricks = 128^3 voxels */
+#define MO_CHUNK 32
+
+/* quad = x:5 | y:5 | z:5 | (w-1):5 | (h-1):5 | face:3 | 0:4 */
+uint32_t mo_quad_pack(int x, int y, int z, int w, int h, int face);
+
+/* ---------------- RNG and hashing ---------------- */
+uint64_t mo_splitmix64(uint64_t *state);
+typedef struct { uint64_t state, inc; } mo_pcg32;
+void mo_pcg32_srandom(mo_pcg32 *r, uint64_t initstate, uint64_t initseq);
+uint32_t mo_pcg32_next(mo_pcg32 *r);
+/* seed -> (initstate, initseq) via two SplitMix64 draws */
+void mo_pcg32_seed(mo_pcg32 *r, uint64_t seed);
+uint32_t mo_hash32(uint32_t x); /* lowbias32 integer finaliser */
+uint32_t mo_lattice_hash3(int32_t x, int32_t y, int32_t z, uint32_t seed);
+float mo_value_noise2(float x, float z, uint32_t seed);
+float mo_value_noise3(float x, float y, float z, uint32_t seed);
+/* fBm in [0,1): octaves, lacunarity 2, gain 0.5, normalised by sum of amps */
+float mo_fbm2(float x, float z, int octaves, float base_freq, uint32_t seed);
+float mo_fbm3(float x, float y, float z, int octaves, float base_freq, uint32_t seed);
+
+/* ---------------- scene generators ---------------- */
+/* dense spheres: params[4*i] = cx, cy, cz, r in 1/256 voxel fixed point */
+void mo_sphere_params(uint64_t seed, int n, int n_spheres, int32_t *params);
+void mo_gen_spheres_dense(const int32_t *params, int n_spheres, int n, uint8_t *grid);
+
+/* heightfield terrain: column height and material rule */
+int32_t mo_height(int32_t x, int32_t z, int32_t n, uint32_t seed);
+uint16_t mo_heightfield_voxel(int32_t x, int32_t y, int32_t z, int32_t n, uint32_t seed);
+/* terrain with carved tunnels (serving scenes) and thin caves (sparse) */
+uint16_t mo_terrain_caves_voxel(int32_t x, int32_t y, int32_t z, int32_t n, uint32_t seed);
+uint16_t mo_caves_voxel(int32_t x, int32_t y, int32_t z, int32_t n, uint32_t seed);
+
+#define MO_GEN_HEIGHTFIELD 1
+#define MO_GEN_TERRAIN_CAVES 2
+#define MO_GEN_CAVES 3
+uint16_t mo_scene_voxel(int kind, int32_t x, int32_t y, int32_t z, int32_t n, uint32_t seed);
+
+/* Brick map of a procedural scene of edge n (multiple of 8).
+ * slots[nb^3] (nb = n/8): pool index or MO_EMPTY_SLOT, assigned in linear
+ * brick order.  masks[8 u64 per brick]: word z, bit x + 8*y.
+ * voxels[512 u16 per brick]: x + 8*y + 64*z.
+ * Call with masks == NULL to count only.  Returns the number of bricks. */
+int64_t mo_build_brickmap(int kind, int32_t n, uint32_t seed, uint32_t *slots,
+                          uint64_t *masks, uint16_t *voxels, int64_t cap_bricks);
+/* Brick map from a dense u16 grid (x fastest). */
+int64_t mo_brickmap_from_dense(const uint16_t *grid, int32_t n, uint32_t *slots,
+                               uint64_t *masks, uint16_t *voxels, int64_t cap_bricks);
+/* Fill one brick (bx,by,bz) of a procedural scene; returns solid count. */
+int32_t mo_gen_brick(int kind, int32_t n, uint32_t seed, int32_t bx, int32_t by,
+                     int32_t bz, uint16_t *vox512, uint64_t *mask8);
+
+/* chunk batch (4096 x 32^3): per-chunk parameters and voxel fill */
+typedef struct {
+  float fill;           /* Beta(1/2,1/2) draw */
+  uint32_t noise_seed;
+  int32_t n_spheres;    /* 1..4 */
+  int32_t sph[4][4];    /* cx, cy, cz, r in 1/256 voxel fixed point */
+} mo_chunk_params;
+void mo_chunk_params_gen(uint64_t seed, int32_t n_chunks, mo_chunk_params *out);
+void mo_gen_chunk(const mo_chunk_params *p, uint16_t *vox32768);
+
+/* ---------------- camera and rays ---------------- */
+typedef struct {
+  float eye[3], fwd[3], right[3], up[3];
+  float sx, sy, inv_w2, inv_h2;
+  int32_t width, height;
+} mo_camera;
+void mo_camera_derive(const float eye[3], const float target[3], const float up[3],
+                      float vfov_deg, int32_t width, int32_t height, mo_camera *cam);
+void mo_primary_ray(const mo_camera *cam, int32_t px, int32_t py, float o[3], float d[3]);
+
+/* ---------------- scenes for traversal ---------------- */
+#define MO_SCENE_DENSE_U8 1
+#define MO_SCENE_BRICKMAP_U16 2
+#define MO_SCENE_TWO_LEVEL_U16 3
+typedef struct {
+  int32_t kind;
+  int32_t n;                 /* voxels per axis (cubic scene) */
+  const uint8_t *dense;      /* dense u8, n^3 */
+  const uint32_t *slots;     /* brick map: (n/8)^3 ; two-level: brick slots of nodes */
+  const uint64_t *masks;
+  const uint16_t *voxels;
+  const uint32_t *top;       /* two-level: (n/128)^3 node indices */
+} mo_scene;
+
+/* step counters for the algorithmic byte model */
+typedef struct {
+  int64_t n_vox, n_top, n_node, n_brickslot, n_brickmask, hits;
+  int64_t rays, shadow_rays, ao_rays;
+} mo_counters;
+
+typedef struct {
+  int64_t index;   /* -1 on miss */
+  int32_t face;    /* 0..5, MO_FACE_INSIDE, MO_FACE_MISS */
+  float t;         /* +inf on miss */
+  int32_t cell[3];
+  uint16_t material;
+} mo_hit;
+
+/* First solid voxel along o + t d inside [0,n)^3.  Origin anywhere. */
+int mo_trace(const mo_scene *s, const float o[3], const float d[3], mo_hit *hit,
+             mo_counters *cnt);
+/* Occlusion ray from a point on a face of voxel `cell`, starting in the empty
+ * neighbour cell `start`.  Returns 1 when a solid voxel is entered at t < tmax. */
+int mo_occluded(const mo_scene *s, const float o[3], const float d[3],
+                const int32_t start[3], float tmax, mo_counters *cnt);
+
+#define MO_RAY_SHADOW 1u
+#define MO_RAY_AO 2u
+typedef struct {
+  float light[3];     /* unit vector toward the light */
+  uint32_t ao_seed;
+  int32_t ao_samples; /* 4 */
+  float ao_radius;    /* voxels */
+} mo_shading;
+/* Renders rows [row0, row1) of the image; out arrays are width*height. */
+void mo_raymarch(const mo_scene *s, const mo_camera *cam, uint32_t flags,
+                 const mo_shading *sh, int32_t row0, int32_t row1,
+                 int64_t *out_index, uint8_t *out_face, float *out_t, float *out_shade,
+                 mo_counters *cnt);
+/* Same over nthreads POSIX threads (static row partition); counters summed. */
+void mo_raymarch_mt(const mo_scene *s, const mo_camera *cam, uint32_t flags,
+                    const mo_shading *sh, int32_t row0, int32_t row1, int32_t nthreads,
+                    int64_t *out_index, uint8_t *out_face, float *out_t, float *out_shade,
+                    mo_counters *cnt);
+/* Cosine-hemisphere AO direction (Malley's method, disk by rejection). */
+void mo_ao_direction(uint32_t seed, uint32_t pixel, uint32_t sample, int face, float dir[3]);
+
+/* ---------------- meshing ---------------- */
+/* chunk of 32^3 voxels, x fastest; elem_size 1 (u8) or 2 (u16); 0 = empty.
+ * Quads are emitted in canonical order: face, slice, v, u.
+ * Returns the quad count (writes at most cap). */
+uint32_t mo_mesh_chunk(const void *vox, int elem_size, int greedy, uint32_t *quads,
+                       uint32_t cap);
+uint64_t mo_mesh_chunks_mt(const void *vox, int elem_size, int32_t n_chunks, int greedy,
+                           int32_t nthreads, uint32_t *counts);
+
+/* ---------------- reference restatements ---------------- */
+/* FChunk::CalculateOccupancyErodeMipmaps (Runtimes/Voxel/Chunk/Chunk.h:73-94)
+ * with FOccupancyHelper::ErodeSingleVoxel (BinaryOccupancyVolume.h:76-99).
+ * occ: res^3 bytes (0/1), x fastest; mips: depth * res^3 bytes. */
+void mo_ref_erode_mips(const uint8_t *occ, int32_t res, int32_t depth, uint8_t *mips);
+/* FGeneratorHelper::GenerateSphere (Runtimes/Helper/GeneratorHelper.h:120-150):
+ * solid cells of one chunk in X-outer, Y, Z-inner order; cells[3*k] = x,y,z. */
+int32_t mo_ref_generate_sphere(int32_t cx, int32_t cy, int32_t cz, float block_size,
+                               int32_t res, uint8_t *cells);
+/* FGeneratorHelper::TestGenerator (GeneratorHelper.h:90-119) with noised /
+ * displacement (:18-87) and FVoxelMathHelper::Hash (VoxelMathHelper.h:29-33).
+ * Parity unpinned: depends on libm sin at large arguments. */
+int32_t mo_ref_test_generator(int32_t cx, int32_t cy, int32_t cz, float block_size,
+                              int32_t res, uint8_t *cells);
+/* GetOctantId (Samples/SimpleVoxel.cpp:129-136) */
+int32_t mo_ref_octant_id(float x, float y, float z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
