@@ -1,0 +1,271 @@
+#!/usr/bin/env python3
+"""Benchmark of the MesoEngine voxel hot path on one MI355X.
+
+Headline (value): Mrays/s of one 1920x1080 frame of primary rays plus one
+shadow ray per lit hit over the 512^3 heightfield brick map (u16 materials,
+8^3 bricks), scene and outputs resident in HBM.  Beside it in the same JSON
+line: config-3 meshing (4096 chunks of 32^3 u16) batched vs plain, serving
+(continuous batching over 16 resident scenes at a fixed QPS, p50/p99) and the
+scalar CPU oracle timed on this host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def raymarch_leg(ctx, R, L, W, steps, warmup):
+    cfg = W.TERRAIN
+    t0 = time.time()
+    sc = ctx.scene_generate(L.GEN_HEIGHTFIELD, L.SCENE_BRICKMAP_U16, cfg["n"], cfg["seed"])
+    ctx.sync()
+    log("scene %d^3: %d bricks, %.1f MB, generated in %.2fs" % (
+        cfg["n"], sc.info().n_bricks, sc.info().device_bytes / 1e6, time.time() - t0))
+    cp = W.terrain_camera()
+    w, h = cfg["width"], cfg["height"]
+    cam = R.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], w, h)
+    sh = R.shading(W.LIGHT)
+    tg = R.RayTargets(ctx, w, h, counters=True)
+    flags = L.RAY_SHADOW
+    # untimed counting pass: the algorithmic byte model needs the step counts
+    R.raymarch(ctx, sc, cam, flags | L.RAY_COUNT, sh, tg)
+    ctx.sync()
+    cnt = tg.counts()
+    for _ in range(warmup):
+        R.raymarch(ctx, sc, cam, flags, sh, tg)
+    ctx.sync()
+    e0, e1 = R.Event(ctx), R.Event(ctx)
+    wall0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        R.raymarch(ctx, sc, cam, flags, sh, tg)
+    e1.record()
+    e1.sync()
+    wall = time.perf_counter() - wall0
+    ms = e0.elapsed_ms(e1) / steps
+    rays = cnt["rays"] + cnt["shadow_rays"]
+    # bytes per frame: 4 B per brick slot visited, 64 B per non-empty brick
+    # mask fetched, 2 B material per hit, 13 B of outputs per pixel
+    alg_bytes = (4 * cnt["n_brickslot"] + 64 * cnt["n_brickmask"] + 2 * cnt["hits"]
+                 + 13 * cnt["rays"])
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    res = dict(ms=ms, wall_ms=wall * 1e3 / steps, rays=rays, primary=cnt["rays"],
+               shadow=cnt["shadow_rays"], counts=cnt, alg_bytes=alg_bytes, achieved_gbs=achieved,
+               mrays=rays / (ms * 1e-3) / 1e6, mrays_primary=cnt["rays"] / (ms * 1e-3) / 1e6)
+    sc.destroy()
+    return res
+
+
+def mesh_leg(ctx, R, L, W, steps, warmup, use_graph=True):
+    n = W.CHUNKS["n_chunks"]
+    vox = ctx.buffer(2 * 32768 * n)
+    L.check(ctx.lib.meso_generate_chunks(ctx.h, W.CHUNKS["seed"], n, vox.ptr, None))
+    src = R.chunk_source(vox, 2, n)
+    out = {}
+    import ctypes as C
+    for mode, name in ((L.MESH_GREEDY, "greedy"), (L.MESH_CULLED, "culled")):
+        # size the quad buffer from a first batched pass
+        probe = R.MeshTargets(ctx, n, 0)
+        R.mesh_chunks(ctx, src, mode, probe)
+        ctx.sync()
+        total = probe.results()[2]
+        mt = R.MeshTargets(ctx, n, total)
+        tune = L.TuneResult()
+        tune.size = C.sizeof(tune)
+        L.check(ctx.lib.meso_autotune_mesh(ctx.h, L.byref(src), mode, mt.counts.ptr, mt.prefix.ptr,
+                                           mt.quads.ptr, mt.capacity, mt.total.ptr, mt.work.ptr,
+                                           L.byref(tune)))
+        g = C.c_void_p()
+        if use_graph:
+            L.check(ctx.lib.meso_mesh_graph_create(ctx.h, L.byref(src), mode, mt.counts.ptr,
+                                                   mt.prefix.ptr, mt.quads.ptr, mt.capacity,
+                                                   mt.total.ptr, mt.work.ptr, C.byref(g)))
+            step = lambda: L.check(ctx.lib.meso_graph_launch(g, None))  # noqa: E731
+        else:
+            step = lambda: R.mesh_chunks(ctx, src, mode, mt)  # noqa: E731
+        for _ in range(warmup):
+            step()
+        e0, e1 = R.Event(ctx), R.Event(ctx)
+        e0.record()
+        for _ in range(steps):
+            step()
+        e1.record()
+        e1.sync()
+        batched_ms = e0.elapsed_ms(e1) / steps
+        counts, prefix, tot, _ = mt.results()
+        if not use_graph:
+            out[name] = dict(batched_ms=batched_ms, quads=tot,
+                             mvox_batched=n * 32768 / (batched_ms * 1e-3) / 1e6)
+            log("mesh %s: batched %.3f ms (direct launch)" % (name, batched_ms))
+            continue
+        # plain: one launch per chunk, captured as one graph of n launches
+        per_cap = int(counts.max()) if n else 1
+        pc = ctx.buffer(4 * n)
+        pq = ctx.buffer(4 * per_cap * n)
+        pg = C.c_void_p()
+        L.check(ctx.lib.meso_mesh_plain_graph_create(ctx.h, L.byref(src), mode, pc.ptr, pq.ptr,
+                                                     per_cap, C.byref(pg)))
+        L.check(ctx.lib.meso_graph_launch(pg, None))
+        ctx.sync()
+        psteps = max(1, min(steps, 5))
+        e0.record()
+        for _ in range(psteps):
+            L.check(ctx.lib.meso_graph_launch(pg, None))
+        e1.record()
+        e1.sync()
+        plain_graph_ms = e0.elapsed_ms(e1) / psteps
+        # plain, launched one call at a time through the API
+        e0.record()
+        for c in range(n):
+            R.mesh_chunk(ctx, vox.ptr + c * 65536, 2, mode, pc.ptr + 4 * c,
+                         pq.ptr + 4 * per_cap * c, per_cap)
+        e1.record()
+        e1.sync()
+        plain_eager_ms = e0.elapsed_ms(e1)
+        pcounts = pc.download(np.uint32)
+        assert np.array_equal(pcounts, counts), "plain and batched counts differ"
+        L.check(ctx.lib.meso_graph_destroy(g))
+        L.check(ctx.lib.meso_graph_destroy(pg))
+        vox_total = n * 32768
+        alg = n * 65536 + 4 * tot + 8 * n
+        out[name] = dict(batched_ms=batched_ms, plain_graph_ms=plain_graph_ms,
+                         plain_eager_ms=plain_eager_ms, quads=tot,
+                         mvox_batched=vox_total / (batched_ms * 1e-3) / 1e6,
+                         mvox_plain=vox_total / (min(plain_graph_ms, plain_eager_ms) * 1e-3) / 1e6,
+                         achieved_gbs=alg / (batched_ms * 1e-3) / 1e9,
+                         tune=dict(grid=tune.grid, best_ms=tune.best_ms, default_ms=tune.default_ms))
+        out[name]["speedup"] = out[name]["mvox_batched"] / out[name]["mvox_plain"]
+        log("mesh %s: batched %.3f ms (%.0f Mvox/s), plain graph %.3f ms, plain eager %.3f ms" % (
+            name, batched_ms, out[name]["mvox_batched"], plain_graph_ms, plain_eager_ms))
+    return out
+
+
+def serve_leg(ctx, R, L, W, n_requests, qps, budget_mb):
+    from mesoengine_amd import serving
+    return serving.run(ctx, n_requests=n_requests, qps=qps, budget_bytes=budget_mb << 20,
+                       log=log)
+
+
+def cpu_leg(W, seconds_cap=30.0):
+    import oracle as O
+    cfg = W.TERRAIN
+    cores = host_cores()
+    t0 = time.time()
+    slots, masks, vox = O.build_brickmap(O.GEN_HEIGHTFIELD, cfg["n"], cfg["seed"])
+    gen_s = time.time() - t0
+    sc = O.brickmap_scene(cfg["n"], slots, masks, vox)
+    cp = W.terrain_camera()
+    cam = O.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], cfg["width"], cfg["height"])
+    sh = O.shading(W.LIGHT)
+    times, rays = [], 0
+    t_start = time.time()
+    while len(times) < 5 and (time.time() - t_start) < seconds_cap:
+        t = time.perf_counter()
+        _, _, _, _, cnt = O.raymarch(sc, cam, O.RAY_SHADOW, sh, threads=cores)
+        times.append(time.perf_counter() - t)
+        rays = cnt.rays + cnt.shadow_rays
+    med = sorted(times)[len(times) // 2]
+    # single-thread on a band of 64 rows
+    t = time.perf_counter()
+    _, _, _, _, c1 = O.raymarch(sc, cam, O.RAY_SHADOW, sh, row0=508, row1=572, threads=1)
+    st = time.perf_counter() - t
+    return dict(value=rays / med / 1e6, unit="Mrays/s", cores=cores, kind="port",
+                sample="full 1920x1080 frame (primary + shadow) of the 512^3 heightfield brick map, "
+                       "median of %d runs on %d threads; scene build %.2fs" % (len(times), cores, gen_s),
+                single_thread_mrays=(c1.rays + c1.shadow_rays) / st / 1e6)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-mesh", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched meshing launched directly instead of by graph replay")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--qps", type=float, default=0.0, help="0 = derive from plain capacity")
+    ap.add_argument("--budget-mb", type=int, default=512)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from mesoengine_amd import _lib as L
+    from mesoengine_amd import runtime as R
+    from mesoengine_amd import workloads as W
+
+    ctx = R.Context(0)
+    info = ctx.device_info()
+    log("device %s, %d CUs" % (info.name.decode(), info.compute_units))
+    rm = raymarch_leg(ctx, R, L, W, args.steps, args.warmup)
+    log("raymarch: %.3f ms/frame, %.1f Mrays/s (%.1f primary), %.0f GB/s algorithmic" % (
+        rm["ms"], rm["mrays"], rm["mrays_primary"], rm["achieved_gbs"]))
+    mesh = None if args.no_mesh else mesh_leg(ctx, R, L, W, args.steps, args.warmup,
+                                                 not args.no_graph)
+    serve = None
+    if not args.no_serve:
+        serve = serve_leg(ctx, R, L, W, args.serve_requests, args.qps, args.budget_mb)
+    cpu = None if args.no_cpu else cpu_leg(W)
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            traffic = json.load(f).get("raymarch_brickmap_terrain512_hbm_bytes_per_launch")
+    line = {
+        "metric": "Mrays/s and p99 latency at fixed QPS; Mvoxels meshed/s; batched vs plain",
+        "value": round(rm["mrays"], 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(rm["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+int32",
+        "data": "synthetic (seeded procedural scenes generated on the GPU)",
+        "config": {"workload": "512^3 heightfield brick map (8^3 bricks, u16), 1920x1080 "
+                               "primary + 1 shadow ray per lit hit", "rays_per_frame": rm["rays"],
+                   "primary_rays": rm["primary"], "shadow_rays": rm["shadow"]},
+        "roofline": {"bound": "hbm", "achieved": round(rm["achieved_gbs"], 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(rm["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "alg_bytes_per_launch": rm["alg_bytes"]},
+        "cpu_baseline": cpu,
+        "mrays_primary": round(rm["mrays_primary"], 2),
+        "meshing": mesh,
+        "serving": serve,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -312,16 +312,19 @@ class Server:
         check(self.ctx.lib.meso_serve_wait(self.h, ticket, timeout_us, C.byref(lat)))
         return lat.value
 
-    def take(self, ticket, width, height, index_bytes=4):
+    def take(self, ticket, width, height, index_bytes=4, parts=("index", "face", "t", "shade")):
+        """Copy the named result arrays to host memory and release the
+        request's HBM budget; arrays not named are returned as None."""
         n = width * height
-        idx = np.empty(n, np.int64 if index_bytes == 8 else np.int32)
-        face = np.empty(n, np.uint8)
-        t = np.empty(n, np.float32)
-        shade = np.empty(n, np.float32)
-        o = L.HostOutputs(C.sizeof(L.HostOutputs), index_bytes, idx.ctypes.data,
-                          face.ctypes.data, t.ctypes.data, shade.ctypes.data)
+        idx = np.empty(n, np.int64 if index_bytes == 8 else np.int32) if "index" in parts else None
+        face = np.empty(n, np.uint8) if "face" in parts else None
+        t = np.empty(n, np.float32) if "t" in parts else None
+        shade = np.empty(n, np.float32) if "shade" in parts else None
+        ptr = lambda a: a.ctypes.data if a is not None else None  # noqa: E731
+        o = L.HostOutputs(C.sizeof(L.HostOutputs), index_bytes, ptr(idx), ptr(face), ptr(t),
+                          ptr(shade))
         check(self.ctx.lib.meso_serve_take(self.h, ticket, byref(o)))
-        return idx.astype(np.int64), face, t, shade
+        return (idx.astype(np.int64) if idx is not None else None), face, t, shade
 
     def stats(self):
         s = L.ServeStats()

@@ -1,0 +1,149 @@
+"""Serving workload: raymarch requests over 16 HBM-resident terrain/cave scenes.
+
+run() measures, on the same seeded request stream:
+  - plain path: one meso_raymarch per request, first-come first-served, each
+    request synchronised before the next starts;
+  - served path: the continuous-batching server (meso_serve_*), admission by
+    an HBM budget for outputs.
+Capacity (Mrays/s) is taken with every request available at t=0; latency
+(p50/p99, arrival -> result ready) at a fixed Poisson QPS, by default 60% of
+the plain path's measured capacity.
+"""
+import threading
+import time
+
+import numpy as np
+
+from . import _lib as L
+from . import runtime as R
+from . import workloads as W
+
+LIT_FACES = (0, 2, 4)  # +Z, +Y, +X face the light (0.4, 1, 0.3)
+
+
+def make_scenes(ctx, edges, seed=4):
+    scenes = []
+    for i, e in enumerate(edges):
+        scenes.append(ctx.scene_generate(L.GEN_TERRAIN_CAVES, L.SCENE_BRICKMAP_U16, e, seed + i))
+    ctx.sync()
+    return scenes
+
+
+def _cam(r):
+    return R.camera(r["eye"], r["target"], r["up"], r["vfov"], r["width"], r["height"])
+
+
+def _rays_of(face):
+    return int(face.size + np.isin(face, LIT_FACES).sum())
+
+
+def plain_run(ctx, scenes, reqs, sh, t0_offsets=None):
+    """FCFS plain path.  With t0_offsets None every request is present at t=0."""
+    wmax = max(r["width"] * r["height"] for r in reqs)
+    tg = R.RayTargets(ctx, wmax, 1)
+    lat, rays = [], 0
+    start = time.perf_counter()
+    for i, r in enumerate(reqs):
+        arrival = start + (r["arrival"] if t0_offsets else 0.0)
+        while time.perf_counter() < arrival:
+            pass
+        cam = _cam(r)
+        R.raymarch(ctx, scenes[r["scene"]], cam, L.RAY_SHADOW, sh, tg)
+        ctx.sync()
+        lat.append(time.perf_counter() - arrival)
+        n = r["width"] * r["height"]
+        face = tg.face.download(np.uint8, n)
+        rays += _rays_of(face)
+    makespan = time.perf_counter() - start
+    return np.array(lat), rays, makespan
+
+
+def served_run(ctx, scenes, reqs, sh, budget_bytes, timed_arrivals, rays_per_launch=1 << 20,
+               max_inflight=2):
+    srv = R.Server(ctx, budget_bytes, max_pending=len(reqs) + 8, max_inflight=max_inflight,
+                   rays_per_launch=rays_per_launch)
+    lat = [None] * len(reqs)
+    rays = [0]
+    tickets = [None] * len(reqs)
+    lag = [0.0] * len(reqs)
+    ready = threading.Semaphore(0)
+    errors = []
+
+    def consumer():
+        try:
+            for i in range(len(reqs)):
+                ready.acquire()
+                r = reqs[i]
+                lt = srv.wait(tickets[i], 60_000_000)
+                lat[i] = lt * 1e-6 + lag[i]
+                # the client reads the face ids, as the plain path does
+                _, face, _, _ = srv.take(tickets[i], r["width"], r["height"], 0,
+                                         parts=("face",))
+                rays[0] += _rays_of(face)
+        except Exception as e:  # surfaced after join
+            errors.append(e)
+
+    th = threading.Thread(target=consumer)
+    th.start()
+    start = time.perf_counter()
+    for i, r in enumerate(reqs):
+        arrival = start + (r["arrival"] if timed_arrivals else 0.0)
+        while time.perf_counter() < arrival:
+            pass
+        t = srv.submit(scenes[r["scene"]], _cam(r), L.RAY_SHADOW, sh)
+        lag[i] = time.perf_counter() - arrival
+        if t is None:
+            raise RuntimeError("server refused a request (pending queue full)")
+        tickets[i] = t
+        ready.release()
+    th.join()
+    makespan = time.perf_counter() - start
+    stats = srv.stats()
+    srv.stop()
+    if errors:
+        raise errors[0]
+    return np.array(lat), rays[0], makespan, stats
+
+
+def _pct(a, q):
+    return float(np.percentile(a, q) * 1e3)
+
+
+def run(ctx, n_requests=400, qps=0.0, budget_bytes=512 << 20, seed=4, log=print):
+    edges = W.serve_scenes(seed)
+    t = time.time()
+    scenes = make_scenes(ctx, edges, seed)
+    mb = sum(s.info().device_bytes for s in scenes) / 1e6
+    log("serving: %d scenes (edges %s), %.0f MB resident, built in %.1fs" % (
+        len(scenes), edges, mb, time.time() - t))
+    sh = R.shading(W.LIGHT)
+    # capacity: everything queued at t=0
+    cap_reqs = W.serve_requests(edges, min(n_requests, 200), 1.0, seed)
+    lat_p, rays_p, mk_p = plain_run(ctx, scenes, cap_reqs, sh, None)
+    lat_s, rays_s, mk_s, st_s = served_run(ctx, scenes, cap_reqs, sh, budget_bytes, False)
+    plain_mrays = rays_p / mk_p / 1e6
+    served_mrays = rays_s / mk_s / 1e6
+    plain_rps = len(cap_reqs) / mk_p
+    if qps <= 0:
+        qps = 0.6 * plain_rps
+    log("serving capacity: plain %.0f Mrays/s (%.0f req/s, %.3fs), served %.0f Mrays/s (%.3fs, "
+        "device busy %.3fs, %d launches); QPS %.0f" % (
+            plain_mrays, plain_rps, mk_p, served_mrays, mk_s, st_s["busy_ms"] / 1e3,
+            st_s["launches"], qps))
+    reqs = W.serve_requests(edges, n_requests, qps, seed)
+    lp, _, _ = plain_run(ctx, scenes, reqs, sh, True)
+    ls, rays_q, mk_q, st_q = served_run(ctx, scenes, reqs, sh, budget_bytes, True)
+    out = dict(qps=round(qps, 1), requests=n_requests, scenes=len(scenes), scene_edges=edges,
+               resident_mb=round(mb, 1), budget_mb=budget_bytes >> 20,
+               capacity_mrays_served=round(served_mrays, 1),
+               capacity_mrays_plain=round(plain_mrays, 1),
+               capacity_ratio=round(served_mrays / plain_mrays, 3),
+               p50_ms_served=round(_pct(ls, 50), 3), p99_ms_served=round(_pct(ls, 99), 3),
+               p50_ms_plain=round(_pct(lp, 50), 3), p99_ms_plain=round(_pct(lp, 99), 3),
+               mrays_at_qps=round(rays_q / mk_q / 1e6, 1),
+               launches=int(st_q["launches"]), peak_budget_mb=round(st_q["peak_bytes_in_use"] / 2**20, 1))
+    log("serving at %.0f QPS: served p50 %.2f p99 %.2f ms; plain p50 %.2f p99 %.2f ms" % (
+        qps, out["p50_ms_served"], out["p99_ms_served"], out["p50_ms_plain"], out["p99_ms_plain"]))
+    for s in scenes:
+        s.destroy()
+    return out

@@ -339,3 +339,45 @@ def ref_test_generator(cx, cy, cz, block_size=1.0, res=16):
     cells = np.zeros((res ** 3, 3), np.uint8)
     k = lib().mo_ref_test_generator(cx, cy, cz, block_size, res, _p(cells))
     return cells[:k]
+
+
+def two_level_from_bricks(n, slots, masks, voxels):
+    """Two-level map (top node = 16^3 bricks) from a brick map of edge n
+    (multiple of 128).  Nodes are numbered in linear top order; bricks are
+    renumbered node-major: node by node, local index bx + 16 by + 256 bz."""
+    nb, nt = n // 8, n // 128
+    s = np.asarray(slots, np.uint32).reshape(nb, nb, nb)  # [bz][by][bx]
+    top = np.full(nt ** 3, EMPTY_SLOT, np.uint32)
+    node_slots, new_masks, new_vox = [], [], []
+    nid, bid = 0, 0
+    for tz in range(nt):
+        for ty in range(nt):
+            for tx in range(nt):
+                blk = s[tz * 16:(tz + 1) * 16, ty * 16:(ty + 1) * 16, tx * 16:(tx + 1) * 16]
+                if (blk == EMPTY_SLOT).all():
+                    continue
+                top[tx + nt * ty + nt * nt * tz] = nid
+                nid += 1
+                flat = blk.reshape(-1)  # local = bx + 16 by + 256 bz
+                tbl = np.full(4096, EMPTY_SLOT, np.uint32)
+                used = np.nonzero(flat != EMPTY_SLOT)[0]
+                tbl[used] = np.arange(bid, bid + len(used), dtype=np.uint32)
+                bid += len(used)
+                new_masks.append(masks[flat[used]])
+                new_vox.append(voxels[flat[used]])
+                node_slots.append(tbl)
+    ns = np.concatenate(node_slots) if node_slots else np.zeros(0, np.uint32)
+    m = np.concatenate(new_masks) if new_masks else np.zeros((0, 8), np.uint64)
+    v = np.concatenate(new_vox) if new_vox else np.zeros((0, 512), np.uint16)
+    return top, ns, m, v
+
+
+def dense_from_bricks(n, slots, voxels):
+    """Dense u16 grid [z][y][x] from a brick map."""
+    nb = n // 8
+    g = np.zeros((n, n, n), np.uint16)
+    s = np.asarray(slots, np.uint32).reshape(nb, nb, nb)
+    for bz, by, bx in zip(*np.nonzero(s != EMPTY_SLOT)):
+        g[bz * 8:bz * 8 + 8, by * 8:by * 8 + 8, bx * 8:bx * 8 + 8] = \
+            voxels[s[bz, by, bx]].reshape(8, 8, 8)
+    return g
