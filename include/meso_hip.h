@@ -216,7 +216,9 @@ typedef struct {
 #define MESO_CNT_RAYS 6
 #define MESO_CNT_SHADOW_RAYS 7
 #define MESO_CNT_AO_RAYS 8
-#define MESO_CNT_COUNT 9
+#define MESO_CNT_N_COARSE 9  /* 32^3 / 128^3 occupancy words read */
+#define MESO_CNT_N_ITER 10   /* traversal steps at any level */
+#define MESO_CNT_COUNT 11
 
 typedef struct {
   uint32_t size;

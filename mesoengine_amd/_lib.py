@@ -17,7 +17,7 @@ RAY_PRIMARY, RAY_SHADOW, RAY_AO, RAY_COUNT = 0, 1, 2, 0x100
 MESH_CULLED, MESH_GREEDY = 0, 1
 CHUNKS_PACKED, CHUNKS_DENSE_GRID = 0, 1
 CNT_NAMES = ["n_vox", "n_top", "n_node", "n_brickslot", "n_brickmask", "hits", "rays",
-             "shadow_rays", "ao_rays"]
+             "shadow_rays", "ao_rays", "n_coarse", "n_iter"]
 
 
 class MesoError(RuntimeError):

@@ -58,6 +58,78 @@ meso_status launch_zero(void *ptr, uint64_t bytes, hipStream_t stream) {
   return MESO_OK;
 }
 
+// ------------------------------------------------------------------ empty-space levels
+// Brick map: one bit per 32^3 and per 128^3 voxel block holding a brick.
+__global__ void coarse_brickmap_kernel(const uint32_t *slots, uint32_t nb, uint32_t *occ32,
+                                       uint32_t *occ128) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)nb * nb * nb || slots[i] == kEmptySlot) return;
+  const uint32_t bx = (uint32_t)(i % nb), by = (uint32_t)((i / nb) % nb),
+                 bz = (uint32_t)(i / ((uint64_t)nb * nb));
+  const uint32_t n32 = (nb + 3) / 4, n128 = (nb + 15) / 16;
+  const uint32_t i32 = (bx >> 2) + ((by >> 2) + (bz >> 2) * n32) * n32;
+  const uint32_t i128 = (bx >> 4) + ((by >> 4) + (bz >> 4) * n128) * n128;
+  atomicOr(&occ32[i32 >> 5], 1u << (i32 & 31u));
+  atomicOr(&occ128[i128 >> 5], 1u << (i128 & 31u));
+}
+
+// Two-level: per node, one bit per 4^3-brick group holding a brick.
+__global__ void coarse_groups_kernel(const uint32_t *node_slots, uint64_t n_nodes,
+                                     unsigned long long *groups) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_nodes * 4096u || node_slots[i] == kEmptySlot) return;
+  const uint32_t l = (uint32_t)(i % 4096u);
+  const uint32_t bx = l & 15u, by = (l >> 4) & 15u, bz = l >> 8;
+  atomicOr(&groups[i / 4096u], 1ull << ((bx >> 2) + 4u * (by >> 2) + 16u * (bz >> 2)));
+}
+
+meso_status build_coarse(meso_scene *sc, hipStream_t stream) {
+  if (sc->kind == MESO_SCENE_BRICKMAP_U16) {
+    const uint64_t nb = (uint64_t)sc->n / 8;
+    const uint64_t n32 = (nb + 3) / 4, n128 = (nb + 15) / 16;
+    const uint64_t w32 = (n32 * n32 * n32 + 31) / 32, w128 = (n128 * n128 * n128 + 31) / 32;
+    MESO_HIP(hipMalloc((void **)&sc->occ32, w32 * 4));
+    MESO_HIP(hipMalloc((void **)&sc->occ128, w128 * 4));
+    sc->bytes += (w32 + w128) * 4;
+    meso_status st = launch_zero(sc->occ32, w32 * 4, stream);
+    if (st == MESO_OK) st = launch_zero(sc->occ128, w128 * 4, stream);
+    if (st != MESO_OK) return st;
+    const uint64_t cells = nb * nb * nb;
+    hipLaunchKernelGGL(coarse_brickmap_kernel, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0,
+                       stream, sc->slots, (uint32_t)nb, sc->occ32, sc->occ128);
+    MESO_HIP(hipGetLastError());
+  } else if (sc->kind == MESO_SCENE_TWO_LEVEL_U16) {
+    const uint64_t nn = (uint64_t)(sc->n_nodes > 0 ? sc->n_nodes : 1);
+    MESO_HIP(hipMalloc((void **)&sc->groups, nn * 8));
+    sc->bytes += nn * 8;
+    meso_status st = launch_zero(sc->groups, nn * 8, stream);
+    if (st != MESO_OK) return st;
+    if (sc->n_nodes > 0) {
+      const uint64_t cells = (uint64_t)sc->n_nodes * 4096u;
+      hipLaunchKernelGGL(coarse_groups_kernel, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0,
+                         stream, sc->slots, (uint64_t)sc->n_nodes,
+                         (unsigned long long *)sc->groups);
+      MESO_HIP(hipGetLastError());
+    }
+  }
+  MESO_HIP(hipStreamSynchronize(stream));
+  return MESO_OK;
+}
+
+void free_scene_arrays(meso_scene *sc) {
+  hipSetDevice(sc->ctx->device);
+  for (void *p : {(void *)sc->dense, (void *)sc->slots, (void *)sc->masks, (void *)sc->voxels,
+                  (void *)sc->top, (void *)sc->occ32, (void *)sc->occ128, (void *)sc->groups})
+    if (p) hipFree(p);
+  sc->dense = nullptr;
+  sc->slots = nullptr;
+  sc->masks = nullptr;
+  sc->voxels = nullptr;
+  sc->top = nullptr;
+  sc->occ32 = sc->occ128 = nullptr;
+  sc->groups = nullptr;
+}
+
 // ------------------------------------------------------------------ RNG
 uint64_t splitmix64(uint64_t *s) {
   uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
@@ -357,12 +429,7 @@ static meso_status scene_alloc(meso_context *ctx, void **p, uint64_t bytes, cons
 
 static void scene_free(meso_scene *sc) {
   if (!sc) return;
-  hipSetDevice(sc->ctx->device);
-  if (sc->dense) hipFree(sc->dense);
-  if (sc->slots) hipFree(sc->slots);
-  if (sc->masks) hipFree(sc->masks);
-  if (sc->voxels) hipFree(sc->voxels);
-  if (sc->top) hipFree(sc->top);
+  free_scene_arrays(sc);
   delete sc;
 }
 
@@ -395,8 +462,8 @@ meso_status meso_scene_create_brickmap_u16(meso_context *ctx, int32_t n, const u
   MESO_CHECK(ctx && slots && out, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
   MESO_CHECK(n >= 8 && n % 8 == 0 && n <= 8192, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
              "brick map edge %d must be a multiple of 8 in 8..8192", n);
-  MESO_CHECK(n_bricks >= 0 && n_bricks < (int64_t)MESO_EMPTY_SLOT, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
-             "n_bricks %lld out of range", (long long)n_bricks);
+  MESO_CHECK(n_bricks >= 0 && n_bricks < (1ll << 29), MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "n_bricks %lld out of range (< 2^29)", (long long)n_bricks);
   MESO_CHECK(n_bricks == 0 || (masks && voxels), MESO_ERR_ARGUMENT_OUT_OF_RANGE,
              "masks/voxels NULL with %lld bricks", (long long)n_bricks);
   const int64_t nb = n / 8, ncell = nb * nb * nb;
@@ -414,6 +481,7 @@ meso_status meso_scene_create_brickmap_u16(meso_context *ctx, int32_t n, const u
   SCENE_TRY(scene_alloc(ctx, (void **)&sc->slots, (uint64_t)ncell * 4, slots, sc));
   SCENE_TRY(scene_alloc(ctx, (void **)&sc->masks, (uint64_t)n_bricks * 64, masks, sc));
   SCENE_TRY(scene_alloc(ctx, (void **)&sc->voxels, (uint64_t)n_bricks * 1024, voxels, sc));
+  SCENE_TRY(build_coarse(sc, ctx->stream));
   *out = sc;
   return MESO_OK;
 }
@@ -423,10 +491,10 @@ meso_status meso_scene_create_two_level_u16(meso_context *ctx, int32_t n, const 
                                             int64_t n_bricks, const uint64_t *masks,
                                             const uint16_t *voxels, meso_scene **out) {
   MESO_CHECK(ctx && top && out, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
-  MESO_CHECK(n >= 128 && n % 128 == 0 && n <= 16384, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
-             "two-level edge %d must be a multiple of 128 in 128..16384", n);
-  MESO_CHECK(n_nodes >= 0 && n_bricks >= 0 && n_bricks < (int64_t)MESO_EMPTY_SLOT,
-             MESO_ERR_ARGUMENT_OUT_OF_RANGE, "negative or oversized counts");
+  MESO_CHECK(n >= 128 && n % 128 == 0 && n <= 8192, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "two-level edge %d must be a multiple of 128 in 128..8192", n);
+  MESO_CHECK(n_nodes >= 0 && n_bricks >= 0 && n_bricks < (1ll << 29),
+             MESO_ERR_ARGUMENT_OUT_OF_RANGE, "negative or oversized counts (bricks < 2^29)");
   MESO_CHECK(n_nodes == 0 || node_slots, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "node_slots NULL");
   MESO_CHECK(n_bricks == 0 || (masks && voxels), MESO_ERR_ARGUMENT_OUT_OF_RANGE,
              "masks/voxels NULL");
@@ -449,6 +517,7 @@ meso_status meso_scene_create_two_level_u16(meso_context *ctx, int32_t n, const 
   SCENE_TRY(scene_alloc(ctx, (void **)&sc->slots, (uint64_t)n_nodes * 4096 * 4, node_slots, sc));
   SCENE_TRY(scene_alloc(ctx, (void **)&sc->masks, (uint64_t)n_bricks * 64, masks, sc));
   SCENE_TRY(scene_alloc(ctx, (void **)&sc->voxels, (uint64_t)n_bricks * 1024, voxels, sc));
+  SCENE_TRY(build_coarse(sc, ctx->stream));
   *out = sc;
   return MESO_OK;
 }

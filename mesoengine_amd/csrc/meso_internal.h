@@ -55,6 +55,9 @@ struct DevScene {
   const uint64_t *masks;
   const uint16_t *voxels;
   const uint32_t *top;
+  const uint32_t *occ32;   // brick map: one bit per 32^3 block holding a brick
+  const uint32_t *occ128;  // brick map: one bit per 128^3 block
+  const uint64_t *groups;  // two-level: per node, one bit per 4^3-brick group
 };
 
 struct RayOut {
@@ -104,11 +107,16 @@ struct meso_scene {
   uint64_t *masks = nullptr;
   uint16_t *voxels = nullptr;
   uint32_t *top = nullptr;
+  uint32_t *occ32 = nullptr;
+  uint32_t *occ128 = nullptr;
+  uint64_t *groups = nullptr;
   int64_t n_bricks = 0;
   int64_t n_nodes = 0;
   uint64_t bytes = 0;
   int64_t solid = -1;
-  meso::DevScene dev() const { return {n, dense, slots, masks, voxels, top}; }
+  meso::DevScene dev() const {
+    return {n, dense, slots, masks, voxels, top, occ32, occ128, groups};
+  }
 };
 
 struct meso_event {
@@ -128,6 +136,9 @@ void derive_camera(const meso_camera *c, DevCamera *out);
 void derive_shading(const meso_shading *s, DevShading *out);
 hipStream_t stream_of(meso_context *ctx, meso_stream s);
 meso_status launch_zero(void *ptr, uint64_t bytes, hipStream_t stream);
+// builds the empty-space levels (occ32/occ128 or node groups) of a brick scene
+meso_status build_coarse(meso_scene *sc, hipStream_t stream);
+void free_scene_arrays(meso_scene *sc);
 
 // kernels launched by other translation units
 meso_status launch_raymarch(meso_context *ctx, const meso_scene *scene, const DevCamera &cam,

@@ -25,7 +25,7 @@ __global__ void __launch_bounds__(TW *TH) raymarch_kernel(DevScene s, DevCamera 
   const int32_t px = (int32_t)(tx * TW + (wave % waves_x) * 8u + (lane & 7u));
   const int32_t py = (int32_t)(ty * TH + (wave / waves_x) * 8u + (lane >> 3));
 
-  StepCounts cnt = {0, 0, 0, 0, 0};
+  StepCounts cnt = {0, 0, 0, 0, 0, 0, 0};
   uint32_t n_shadow = 0, n_ao = 0;
   bool valid = px < cam.width && py < cam.height;
   PixelResult r{-1, kFaceMiss, 0.0f, 0.0f};
@@ -54,6 +54,8 @@ __global__ void __launch_bounds__(TW *TH) raymarch_kernel(DevScene s, DevCamera 
       atomicAdd(&acc[MESO_CNT_RAYS], 1ull);
       atomicAdd(&acc[MESO_CNT_SHADOW_RAYS], (unsigned long long)n_shadow);
       atomicAdd(&acc[MESO_CNT_AO_RAYS], (unsigned long long)n_ao);
+      atomicAdd(&acc[MESO_CNT_N_COARSE], (unsigned long long)cnt.n_coarse);
+      atomicAdd(&acc[MESO_CNT_N_ITER], (unsigned long long)cnt.n_iter);
     }
     __syncthreads();
     if (threadIdx.x < MESO_CNT_COUNT && out.counters)

@@ -49,7 +49,7 @@ struct WorkItem {
 
 template <int KIND>
 __device__ __forceinline__ void serve_pixel(const ServeReqDev &r, int32_t px, int32_t py) {
-  StepCounts cnt = {0, 0, 0, 0, 0};
+  StepCounts cnt = {0, 0, 0, 0, 0, 0, 0};
   uint32_t ns = 0, na = 0;
   const PixelResult res = shade_pixel<KIND, false>(r.scene, r.cam, r.flags, r.sh, px, py, cnt, ns, na);
   const int64_t pix = (int64_t)py * r.cam.width + px;

@@ -1,9 +1,10 @@
-// Device-side voxel traversal for gfx950 (Amanatides & Woo 1987 DDA with
-// hierarchical empty-space skipping).  Every float operation is written out in
-// a fixed order and the library is compiled with -ffp-contract=off, so results
-// are the IEEE binary32 results of these operations: the integer outputs
-// (voxel index, face) and the float outputs (t, shade) are reproducible
-// bit-for-bit by any scalar IEEE implementation of the same steps.
+// Device-side voxel traversal for gfx950: Amanatides & Woo (1987) DDA with
+// hierarchical empty-space skipping over 128^3 blocks, 32^3 blocks, 8^3
+// bricks and voxels.  Every float operation is written out in a fixed order
+// and the library is compiled with -ffp-contract=off, so results are the IEEE
+// binary32 results of these operations: the integer outputs (voxel index,
+// face) and the float outputs (t, shade) are reproducible bit for bit by any
+// scalar IEEE implementation of the same steps.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -52,11 +53,6 @@ __device__ __forceinline__ float plane_t(float o, float inv, float b) {
   float q = b - o;
   return q * inv;
 }
-__device__ __forceinline__ float next_t(const Ray &r, int a, int32_t c) {
-  if (r.step[a] > 0) return plane_t(r.o[a], r.inv[a], (float)(c + 1));
-  if (r.step[a] < 0) return plane_t(r.o[a], r.inv[a], (float)c);
-  return __builtin_inff();
-}
 // smallest t, ties to x then y
 __device__ __forceinline__ int argmin3(float t0, float t1, float t2) {
   if (t0 <= t1 && t0 <= t2) return 0;
@@ -70,136 +66,166 @@ struct WalkState {
 };
 
 struct StepCounts {
-  uint32_t n_vox, n_top, n_node, n_brickslot, n_brickmask;
+  uint32_t n_vox, n_top, n_node, n_brickslot, n_brickmask, n_coarse, n_iter;
 };
 
 __device__ __forceinline__ int64_t lin3(int64_t x, int64_t y, int64_t z, int64_t n) {
   return x + y * n + z * n * n;
 }
+__device__ __forceinline__ uint32_t lin3u(uint32_t x, uint32_t y, uint32_t z, uint32_t n) {
+  return x + (y + z * n) * n;
+}
 
-// Leave the empty block (edge L) containing the current cell through its
-// nearest plane.  The other two coordinates come from the exit point, clamped
-// into the block and never moving against the ray.
-template <int L>
-__device__ __forceinline__ bool skip_block(const Ray &r, int32_t n, const int32_t bc[3],
-                                           WalkState &w, float tn[3]) {
+// One step at level 2^sh (sh = 0 voxel, 3 brick, 5 and 7 empty blocks): leave
+// the block holding the current cell through its nearest plane.  For sh > 0
+// the other two coordinates come from the exit point, clamped into the block
+// and the scene and never moving against the ray; for sh = 0 they stay, which
+// is the plain voxel step.  Returns false when the ray leaves the scene.
+__device__ __forceinline__ bool advance(const Ray &r, int32_t n, int32_t sh, WalkState &w) {
   float tb[3];
+  int32_t bc[3];
 #pragma unroll
   for (int a = 0; a < 3; a++) {
+    bc[a] = w.cell[a] >> sh;
     if (r.step[a] > 0)
-      tb[a] = plane_t(r.o[a], r.inv[a], (float)((bc[a] + 1) * L));
+      tb[a] = plane_t(r.o[a], r.inv[a], (float)((bc[a] + 1) << sh));
     else if (r.step[a] < 0)
-      tb[a] = plane_t(r.o[a], r.inv[a], (float)(bc[a] * L));
+      tb[a] = plane_t(r.o[a], r.inv[a], (float)(bc[a] << sh));
     else
       tb[a] = __builtin_inff();
   }
   const int ax = argmin3(tb[0], tb[1], tb[2]);
   float t = tb[0];
-  int32_t na = 0;
-  int32_t st = 0;
+  int32_t na = 0, st = 0;
 #pragma unroll
   for (int a = 0; a < 3; a++)
     if (a == ax) {
       t = tb[a];
       st = r.step[a];
-      na = st > 0 ? (bc[a] + 1) * L : bc[a] * L - 1;
+      na = st > 0 ? (bc[a] + 1) << sh : (bc[a] << sh) - 1;
     }
   if (na < 0 || na >= n) return false;
+  if (sh > 0) {
 #pragma unroll
-  for (int b = 0; b < 3; b++) {
-    if (b == ax) {
-      w.cell[b] = na;
-      continue;
+    for (int b = 0; b < 3; b++) {
+      if (b == ax) continue;
+      float dt = r.d[b] * t;
+      float p = r.o[b] + dt;
+      int32_t c = (int32_t)__builtin_floorf(p);
+      int32_t lo = bc[b] << sh, hi = lo + (1 << sh) - 1;
+      hi = hi > n - 1 ? n - 1 : hi;
+      c = c < lo ? lo : c;
+      c = c > hi ? hi : c;
+      if (r.step[b] > 0 && c < w.cell[b]) c = w.cell[b];
+      if (r.step[b] < 0 && c > w.cell[b]) c = w.cell[b];
+      w.cell[b] = c;
     }
-    float dt = r.d[b] * t;
-    float p = r.o[b] + dt;
-    int32_t c = (int32_t)__builtin_floorf(p);
-    int32_t lo = bc[b] * L, hi = lo + L - 1;
-    c = c < lo ? lo : c;
-    c = c > hi ? hi : c;
-    if (r.step[b] > 0 && c < w.cell[b]) c = w.cell[b];
-    if (r.step[b] < 0 && c > w.cell[b]) c = w.cell[b];
-    w.cell[b] = c;
   }
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+    if (a == ax) w.cell[a] = na;
   w.t = t;
   w.face = entry_face(ax, st);
-#pragma unroll
-  for (int a = 0; a < 3; a++) tn[a] = next_t(r, a, w.cell[a]);
   return true;
 }
 
+__device__ __forceinline__ bool occ_bit(const uint32_t *bits, uint32_t i) {
+  return (bits[i >> 5] >> (i & 31u)) & 1u;
+}
+
 // Walk until a solid voxel is entered (true), the ray leaves the scene or the
-// entry t reaches tmax (false).  KIND: 1 dense u8, 2 brick map, 3 two-level.
+// entry t reaches tmax (false).  KIND: 1 dense u8 (voxel steps only), 2 brick
+// map, 3 two-level.  Each step of a brick scene takes the coarsest empty
+// level holding the cell: 128^3 block, 32^3 block, brick, else the voxel.
 template <int KIND, bool COUNT>
 __device__ __forceinline__ bool walk(const DevScene &s, const Ray &r, WalkState &w, float tmax,
                                      uint16_t &material, StepCounts &cnt) {
   const int32_t n = s.n;
-  float tn[3];
-#pragma unroll
-  for (int a = 0; a < 3; a++) tn[a] = next_t(r, a, w.cell[a]);
-  int32_t cur_node[3] = {-1, -1, -1};
-  int32_t cur_brick[3] = {-1, -1, -1};
-  uint32_t node = kEmpty, slot = kEmpty;
-  for (;;) {
-    if (!(w.t < tmax)) return false;
-    bool solid;
-    if constexpr (KIND == 1) {
-      if (COUNT) cnt.n_vox++;
-      uint8_t v = s.dense[lin3(w.cell[0], w.cell[1], w.cell[2], n)];
-      solid = v != 0;
-      if (solid) material = v;
-    } else {
-      if constexpr (KIND == 3) {
-        int32_t nc[3] = {w.cell[0] >> 7, w.cell[1] >> 7, w.cell[2] >> 7};
-        if (nc[0] != cur_node[0] || nc[1] != cur_node[1] || nc[2] != cur_node[2]) {
-          cur_node[0] = nc[0];
-          cur_node[1] = nc[1];
-          cur_node[2] = nc[2];
-          cur_brick[0] = cur_brick[1] = cur_brick[2] = -1;
-          if (COUNT) cnt.n_top++;
-          node = s.top[lin3(nc[0], nc[1], nc[2], n >> 7)];
-        }
-        if (node == kEmpty) {
-          if (!skip_block<128>(r, n, nc, w, tn)) return false;
-          continue;
-        }
+  if constexpr (KIND == 1) {
+    for (;;) {
+      if (!(w.t < tmax)) return false;
+      if (COUNT) {
+        cnt.n_iter++;
+        cnt.n_vox++;
       }
-      int32_t bc[3] = {w.cell[0] >> 3, w.cell[1] >> 3, w.cell[2] >> 3};
-      if (bc[0] != cur_brick[0] || bc[1] != cur_brick[1] || bc[2] != cur_brick[2]) {
-        cur_brick[0] = bc[0];
-        cur_brick[1] = bc[1];
-        cur_brick[2] = bc[2];
-        if constexpr (KIND == 3) {
-          if (COUNT) cnt.n_node++;
-          slot = s.slots[(int64_t)node * 4096 + lin3(bc[0] & 15, bc[1] & 15, bc[2] & 15, 16)];
-        } else {
-          if (COUNT) cnt.n_brickslot++;
-          slot = s.slots[lin3(bc[0], bc[1], bc[2], n >> 3)];
-        }
-        if (COUNT && slot != kEmpty) cnt.n_brickmask++;
+      const uint8_t v = s.dense[lin3(w.cell[0], w.cell[1], w.cell[2], n)];
+      if (v) {
+        material = v;
+        return true;
       }
-      if (slot == kEmpty) {
-        if (!skip_block<8>(r, n, bc, w, tn)) return false;
-        continue;
-      }
-      const int32_t lx = w.cell[0] & 7, ly = w.cell[1] & 7, lz = w.cell[2] & 7;
-      const uint64_t m = s.masks[(int64_t)slot * 8 + lz];
-      solid = ((m >> (lx + 8 * ly)) & 1ull) != 0;
-      if (solid) material = s.voxels[(int64_t)slot * 512 + lx + 8 * ly + 64 * lz];
+      if (!advance(r, n, 0, w)) return false;
     }
-    if (solid) return true;
-    const int ax = argmin3(tn[0], tn[1], tn[2]);
-    bool out = false;
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-      if (a == ax) {
-        w.t = tn[a];
-        w.cell[a] += r.step[a];
-        out = w.cell[a] < 0 || w.cell[a] >= n;
-        w.face = entry_face(a, r.step[a]);
-        tn[a] = next_t(r, a, w.cell[a]);
+  } else {
+    const uint32_t n128 = (uint32_t)(n + 127) >> 7, n32 = (uint32_t)(n + 31) >> 5;
+    int32_t c128 = -1, c32 = -1, cb = -1;  // packed block keys of the cached levels
+    bool e128 = false, e32 = false;
+    uint32_t node = kEmpty, slot = kEmpty;
+    uint64_t groups = 0;
+    for (;;) {
+      if (!(w.t < tmax)) return false;
+      if (COUNT) cnt.n_iter++;
+      int32_t sh = 0;
+      const uint32_t x = (uint32_t)w.cell[0], y = (uint32_t)w.cell[1], z = (uint32_t)w.cell[2];
+      const int32_t k128 = (int32_t)((x >> 7) | ((y >> 7) << 10) | ((z >> 7) << 20));
+      if (k128 != c128) {
+        c128 = k128;
+        c32 = -1;
+        cb = -1;
+        if constexpr (KIND == 3) {
+          if (COUNT) cnt.n_top++;
+          node = s.top[lin3u(x >> 7, y >> 7, z >> 7, n128)];
+          e128 = node == kEmpty;
+          groups = e128 ? 0ull : s.groups[node];
+        } else {
+          if (COUNT) cnt.n_coarse++;
+          e128 = !occ_bit(s.occ128, lin3u(x >> 7, y >> 7, z >> 7, n128));
+        }
       }
-    if (out) return false;
+      if (e128) {
+        sh = 7;
+      } else {
+        const int32_t k32 = (int32_t)((x >> 5) | ((y >> 5) << 10) | ((z >> 5) << 20));
+        if (k32 != c32) {
+          c32 = k32;
+          cb = -1;
+          if constexpr (KIND == 3) {
+            e32 = !((groups >> (((x >> 5) & 3u) + 4u * ((y >> 5) & 3u) + 16u * ((z >> 5) & 3u))) &
+                    1ull);
+          } else {
+            if (COUNT) cnt.n_coarse++;
+            e32 = !occ_bit(s.occ32, lin3u(x >> 5, y >> 5, z >> 5, n32));
+          }
+        }
+        if (e32) {
+          sh = 5;
+        } else {
+          const int32_t kb = (int32_t)((x >> 3) | ((y >> 3) << 11) | ((z >> 3) << 22));
+          if (kb != cb) {
+            cb = kb;
+            if constexpr (KIND == 3) {
+              if (COUNT) cnt.n_node++;
+              slot = s.slots[(uint64_t)node * 4096u +
+                             lin3u((x >> 3) & 15u, (y >> 3) & 15u, (z >> 3) & 15u, 16u)];
+            } else {
+              if (COUNT) cnt.n_brickslot++;
+              slot = s.slots[lin3u(x >> 3, y >> 3, z >> 3, (uint32_t)n >> 3)];
+            }
+            if (COUNT && slot != kEmpty) cnt.n_brickmask++;
+          }
+          if (slot == kEmpty) {
+            sh = 3;
+          } else {
+            const uint32_t lx = x & 7u, ly = y & 7u, lz = z & 7u;
+            const uint64_t m = s.masks[(uint64_t)slot * 8u + lz];
+            if ((m >> (lx + 8u * ly)) & 1ull) {
+              material = s.voxels[(uint64_t)slot * 512u + lx + 8u * ly + 64u * lz];
+              return true;
+            }
+          }
+        }
+      }
+      if (!advance(r, n, sh, w)) return false;
+    }
   }
 }
 

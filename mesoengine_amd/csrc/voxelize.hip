@@ -354,11 +354,7 @@ static meso_status gen_free(meso_scene *sc, meso_status st) {
   if (sc) {
     hipSetDevice(sc->ctx->device);
     hipStreamSynchronize(sc->ctx->stream);
-    if (sc->dense) hipFree(sc->dense);
-    if (sc->slots) hipFree(sc->slots);
-    if (sc->masks) hipFree(sc->masks);
-    if (sc->voxels) hipFree(sc->voxels);
-    if (sc->top) hipFree(sc->top);
+    free_scene_arrays(sc);
     delete sc;
   }
   return st;
@@ -502,6 +498,9 @@ meso_status meso_scene_generate(meso_context *ctx, const meso_generate_desc *d, 
   sc->n_bricks = (int64_t)n_bricks;
   sc->n_nodes = (int64_t)n_nodes;
   sc->solid = (int64_t)s;
+  MESO_CHECK(n_bricks < (1ull << 29), MESO_ERR_OUT_OF_MEMORY, "%llu bricks exceed 2^29",
+             (unsigned long long)n_bricks);
+  GEN_TRY(build_coarse(sc, stream));
   *out = sc;
   return MESO_OK;
 }

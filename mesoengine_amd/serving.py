@@ -1,14 +1,17 @@
 """Serving workload: raymarch requests over 16 HBM-resident terrain/cave scenes.
 
 run() measures, on the same seeded request stream:
-  - plain path: one meso_raymarch per request, first-come first-served, each
+  - plain path: one meso_raymarch per request, first come first served, each
     request synchronised before the next starts;
   - served path: the continuous-batching server (meso_serve_*), admission by
     an HBM budget for outputs.
-Capacity (Mrays/s) is taken with every request available at t=0; latency
-(p50/p99, arrival -> result ready) at a fixed Poisson QPS, by default 60% of
-the plain path's measured capacity.
+In both paths the client reads the face ids of each result (1 byte/pixel).
+Capacity (Mrays/s) is taken with every request queued at t=0; latency (p50/p99,
+arrival -> result ready on the device) at a fixed Poisson QPS, by default 60%
+of the plain path's measured capacity.  Rays = primary rays + shadow rays,
+counted once per request in an untimed pre-pass.
 """
+import sys
 import threading
 import time
 
@@ -22,9 +25,8 @@ LIT_FACES = (0, 2, 4)  # +Z, +Y, +X face the light (0.4, 1, 0.3)
 
 
 def make_scenes(ctx, edges, seed=4):
-    scenes = []
-    for i, e in enumerate(edges):
-        scenes.append(ctx.scene_generate(L.GEN_TERRAIN_CAVES, L.SCENE_BRICKMAP_U16, e, seed + i))
+    scenes = [ctx.scene_generate(L.GEN_TERRAIN_CAVES, L.SCENE_BRICKMAP_U16, e, seed + i)
+              for i, e in enumerate(edges)]
     ctx.sync()
     return scenes
 
@@ -33,29 +35,43 @@ def _cam(r):
     return R.camera(r["eye"], r["target"], r["up"], r["vfov"], r["width"], r["height"])
 
 
-def _rays_of(face):
-    return int(face.size + np.isin(face, LIT_FACES).sum())
+def _wait_until(t):
+    """Sleep until shortly before t, then spin: the waiting thread must not
+    hold the interpreter lock while the result consumer runs."""
+    while True:
+        left = t - time.perf_counter()
+        if left <= 0:
+            return
+        if left > 3e-4:
+            time.sleep(left - 2e-4)
 
 
-def plain_run(ctx, scenes, reqs, sh, t0_offsets=None):
-    """FCFS plain path.  With t0_offsets None every request is present at t=0."""
+def count_rays(ctx, scenes, reqs, sh):
     wmax = max(r["width"] * r["height"] for r in reqs)
     tg = R.RayTargets(ctx, wmax, 1)
-    lat, rays = [], 0
+    out = []
+    for r in reqs:
+        R.raymarch(ctx, scenes[r["scene"]], _cam(r), L.RAY_SHADOW, sh, tg)
+        ctx.sync()
+        face = tg.face.download(np.uint8, r["width"] * r["height"])
+        bc = np.bincount(face, minlength=256)
+        out.append(int(face.size + bc[list(LIT_FACES)].sum()))
+    return out
+
+
+def plain_run(ctx, scenes, reqs, sh, timed_arrivals):
+    wmax = max(r["width"] * r["height"] for r in reqs)
+    tg = R.RayTargets(ctx, wmax, 1)
+    lat = []
     start = time.perf_counter()
-    for i, r in enumerate(reqs):
-        arrival = start + (r["arrival"] if t0_offsets else 0.0)
-        while time.perf_counter() < arrival:
-            pass
-        cam = _cam(r)
-        R.raymarch(ctx, scenes[r["scene"]], cam, L.RAY_SHADOW, sh, tg)
+    for r in reqs:
+        arrival = start + (r["arrival"] if timed_arrivals else 0.0)
+        _wait_until(arrival)
+        R.raymarch(ctx, scenes[r["scene"]], _cam(r), L.RAY_SHADOW, sh, tg)
         ctx.sync()
         lat.append(time.perf_counter() - arrival)
-        n = r["width"] * r["height"]
-        face = tg.face.download(np.uint8, n)
-        rays += _rays_of(face)
-    makespan = time.perf_counter() - start
-    return np.array(lat), rays, makespan
+        tg.face.download(np.uint8, r["width"] * r["height"])
+    return np.array(lat), time.perf_counter() - start
 
 
 def served_run(ctx, scenes, reqs, sh, budget_bytes, timed_arrivals, rays_per_launch=1 << 20,
@@ -63,7 +79,6 @@ def served_run(ctx, scenes, reqs, sh, budget_bytes, timed_arrivals, rays_per_lau
     srv = R.Server(ctx, budget_bytes, max_pending=len(reqs) + 8, max_inflight=max_inflight,
                    rays_per_launch=rays_per_launch)
     lat = [None] * len(reqs)
-    rays = [0]
     tickets = [None] * len(reqs)
     lag = [0.0] * len(reqs)
     ready = threading.Semaphore(0)
@@ -76,33 +91,31 @@ def served_run(ctx, scenes, reqs, sh, budget_bytes, timed_arrivals, rays_per_lau
                 r = reqs[i]
                 lt = srv.wait(tickets[i], 60_000_000)
                 lat[i] = lt * 1e-6 + lag[i]
-                # the client reads the face ids, as the plain path does
-                _, face, _, _ = srv.take(tickets[i], r["width"], r["height"], 0,
-                                         parts=("face",))
-                rays[0] += _rays_of(face)
+                srv.take(tickets[i], r["width"], r["height"], 0, parts=("face",))
         except Exception as e:  # surfaced after join
             errors.append(e)
 
     th = threading.Thread(target=consumer)
     th.start()
     start = time.perf_counter()
-    for i, r in enumerate(reqs):
-        arrival = start + (r["arrival"] if timed_arrivals else 0.0)
-        while time.perf_counter() < arrival:
-            pass
-        t = srv.submit(scenes[r["scene"]], _cam(r), L.RAY_SHADOW, sh)
-        lag[i] = time.perf_counter() - arrival
-        if t is None:
-            raise RuntimeError("server refused a request (pending queue full)")
-        tickets[i] = t
-        ready.release()
-    th.join()
+    try:
+        for i, r in enumerate(reqs):
+            arrival = start + (r["arrival"] if timed_arrivals else 0.0)
+            _wait_until(arrival)
+            t = srv.submit(scenes[r["scene"]], _cam(r), L.RAY_SHADOW, sh)
+            lag[i] = time.perf_counter() - arrival
+            if t is None:
+                raise RuntimeError("server refused a request (pending queue full)")
+            tickets[i] = t
+            ready.release()
+    finally:
+        th.join()
     makespan = time.perf_counter() - start
     stats = srv.stats()
     srv.stop()
     if errors:
         raise errors[0]
-    return np.array(lat), rays[0], makespan, stats
+    return np.array(lat), makespan, stats
 
 
 def _pct(a, q):
@@ -110,6 +123,7 @@ def _pct(a, q):
 
 
 def run(ctx, n_requests=400, qps=0.0, budget_bytes=512 << 20, seed=4, log=print):
+    sys.setswitchinterval(1e-4)
     edges = W.serve_scenes(seed)
     t = time.time()
     scenes = make_scenes(ctx, edges, seed)
@@ -119,10 +133,11 @@ def run(ctx, n_requests=400, qps=0.0, budget_bytes=512 << 20, seed=4, log=print)
     sh = R.shading(W.LIGHT)
     # capacity: everything queued at t=0
     cap_reqs = W.serve_requests(edges, min(n_requests, 200), 1.0, seed)
-    lat_p, rays_p, mk_p = plain_run(ctx, scenes, cap_reqs, sh, None)
-    lat_s, rays_s, mk_s, st_s = served_run(ctx, scenes, cap_reqs, sh, budget_bytes, False)
-    plain_mrays = rays_p / mk_p / 1e6
-    served_mrays = rays_s / mk_s / 1e6
+    cap_rays = sum(count_rays(ctx, scenes, cap_reqs, sh))
+    _, mk_p = plain_run(ctx, scenes, cap_reqs, sh, False)
+    _, mk_s, st_s = served_run(ctx, scenes, cap_reqs, sh, budget_bytes, False)
+    plain_mrays = cap_rays / mk_p / 1e6
+    served_mrays = cap_rays / mk_s / 1e6
     plain_rps = len(cap_reqs) / mk_p
     if qps <= 0:
         qps = 0.6 * plain_rps
@@ -131,8 +146,9 @@ def run(ctx, n_requests=400, qps=0.0, budget_bytes=512 << 20, seed=4, log=print)
             plain_mrays, plain_rps, mk_p, served_mrays, mk_s, st_s["busy_ms"] / 1e3,
             st_s["launches"], qps))
     reqs = W.serve_requests(edges, n_requests, qps, seed)
-    lp, _, _ = plain_run(ctx, scenes, reqs, sh, True)
-    ls, rays_q, mk_q, st_q = served_run(ctx, scenes, reqs, sh, budget_bytes, True)
+    rays_q = sum(count_rays(ctx, scenes, reqs, sh))
+    lp, _ = plain_run(ctx, scenes, reqs, sh, True)
+    ls, mk_q, st_q = served_run(ctx, scenes, reqs, sh, budget_bytes, True)
     out = dict(qps=round(qps, 1), requests=n_requests, scenes=len(scenes), scene_edges=edges,
                resident_mb=round(mb, 1), budget_mb=budget_bytes >> 20,
                capacity_mrays_served=round(served_mrays, 1),
@@ -141,7 +157,8 @@ def run(ctx, n_requests=400, qps=0.0, budget_bytes=512 << 20, seed=4, log=print)
                p50_ms_served=round(_pct(ls, 50), 3), p99_ms_served=round(_pct(ls, 99), 3),
                p50_ms_plain=round(_pct(lp, 50), 3), p99_ms_plain=round(_pct(lp, 99), 3),
                mrays_at_qps=round(rays_q / mk_q / 1e6, 1),
-               launches=int(st_q["launches"]), peak_budget_mb=round(st_q["peak_bytes_in_use"] / 2**20, 1))
+               launches=int(st_q["launches"]),
+               peak_budget_mb=round(st_q["peak_bytes_in_use"] / 2**20, 1))
     log("serving at %.0f QPS: served p50 %.2f p99 %.2f ms; plain p50 %.2f p99 %.2f ms" % (
         qps, out["p50_ms_served"], out["p99_ms_served"], out["p50_ms_plain"], out["p99_ms_plain"]))
     for s in scenes:

@@ -36,13 +36,14 @@ class Camera(C.Structure):
 class Scene(C.Structure):
     _fields_ = [("kind", C.c_int32), ("n", C.c_int32), ("dense", C.c_void_p),
                 ("slots", C.c_void_p), ("masks", C.c_void_p), ("voxels", C.c_void_p),
-                ("top", C.c_void_p)]
+                ("top", C.c_void_p), ("occ32", C.c_void_p), ("occ128", C.c_void_p),
+                ("groups", C.c_void_p)]
 
 
 class Counters(C.Structure):
     _fields_ = [(k, C.c_int64) for k in ("n_vox", "n_top", "n_node", "n_brickslot",
                                           "n_brickmask", "hits", "rays", "shadow_rays",
-                                          "ao_rays")]
+                                          "ao_rays", "n_coarse", "n_iter")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -105,6 +106,7 @@ def lib():
             "mo_ref_generate_sphere": (C.c_int32, [C.c_int32] * 3 + [C.c_float, C.c_int32, P]),
             "mo_ref_test_generator": (C.c_int32, [C.c_int32] * 3 + [C.c_float, C.c_int32, P]),
             "mo_ref_octant_id": (C.c_int32, [C.c_float] * 3),
+            "mo_build_coarse": (None, [P, C.c_int64, P, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -214,12 +216,26 @@ def primary_ray(cam, px, py):
 
 
 class OracleScene:
-    """Keeps the numpy arrays alive behind an mo_scene struct."""
+    """Keeps the numpy arrays alive behind an mo_scene struct; builds the
+    empty-space levels (occ32 / occ128 or node groups) of brick scenes."""
 
     def __init__(self, kind, n, dense=None, slots=None, masks=None, voxels=None, top=None):
         self.arrays = [a for a in (dense, slots, masks, voxels, top) if a is not None]
-        self.s = Scene(kind, n, _p(dense), _p(slots), _p(masks), _p(voxels), _p(top))
+        self.s = Scene(kind, n, _p(dense), _p(slots), _p(masks), _p(voxels), _p(top),
+                       None, None, None)
         self.kind, self.n = kind, n
+        self.occ32 = self.occ128 = self.groups = None
+        if kind == SCENE_BRICKMAP_U16:
+            n32, n128 = -(-n // 32), -(-n // 128)
+            self.occ32 = np.zeros((n32 ** 3 + 31) // 32, np.uint32)
+            self.occ128 = np.zeros((n128 ** 3 + 31) // 32, np.uint32)
+            self.s.occ32, self.s.occ128 = _p(self.occ32), _p(self.occ128)
+            lib().mo_build_coarse(self.ptr, 0, _p(self.occ32), _p(self.occ128), None)
+        elif kind == SCENE_TWO_LEVEL_U16:
+            n_nodes = len(slots) // 4096
+            self.groups = np.zeros(max(n_nodes, 1), np.uint64)
+            self.s.groups = _p(self.groups)
+            lib().mo_build_coarse(self.ptr, n_nodes, None, None, _p(self.groups))
 
     @property
     def ptr(self):

@@ -504,11 +504,6 @@ static float plane_t(const ray_t *r, int a, float b) {
   float q = b - r->o[a];
   return q * r->inv[a];
 }
-static float next_t(const ray_t *r, int a, int32_t c) {
-  if (r->step[a] > 0) return plane_t(r, a, (float)(c + 1));
-  if (r->step[a] < 0) return plane_t(r, a, (float)c);
-  return INFINITY;
-}
 /* smallest t; ties go to x, then y (the x<y<z rule) */
 static int argmin3(const float t[3]) {
   if (t[0] <= t[1] && t[0] <= t[2]) return 0;
@@ -521,14 +516,20 @@ typedef struct {
   int32_t face;
 } walk_state;
 
-/* Jump over an empty block of edge L (node or brick) containing the current
- * cell: leave it through the nearest block plane, re-derive the other two
- * cell coordinates from the exit point, clamped into the block and never
- * moving against the ray.  Returns 0 when the ray leaves the scene. */
-static int skip_block(const ray_t *r, int32_t n, int32_t L, const int32_t bc[3], walk_state *w,
-                      float tn[3]) {
+static int get_bit(const uint32_t *bits, int64_t i) { return (int)((bits[i >> 5] >> (i & 31)) & 1u); }
+static int32_t ceil_div(int32_t a, int32_t b) { return (a + b - 1) / b; }
+
+/* One traversal step at level L (1 = voxel, 8 = brick, 32, 128 = empty
+ * blocks): leave the block of edge L that holds the current cell through its
+ * nearest plane (ties x, then y).  For L > 1 the other two coordinates come
+ * from the exit point, clamped into the block (and into the scene) and never
+ * moving against the ray; for L = 1 they are unchanged, so the step is the
+ * plain Amanatides & Woo voxel step.  Returns 0 when the ray leaves. */
+static int advance(const ray_t *r, int32_t n, int32_t L, walk_state *w) {
+  int32_t bc[3];
   float tb[3];
   for (int a = 0; a < 3; a++) {
+    bc[a] = w->cell[a] / L;
     if (r->step[a] > 0) tb[a] = plane_t(r, a, (float)((bc[a] + 1) * L));
     else if (r->step[a] < 0) tb[a] = plane_t(r, a, (float)(bc[a] * L));
     else tb[a] = INFINITY;
@@ -537,87 +538,140 @@ static int skip_block(const ray_t *r, int32_t n, int32_t L, const int32_t bc[3],
   float t = tb[a];
   int32_t na = r->step[a] > 0 ? (bc[a] + 1) * L : bc[a] * L - 1;
   if (na < 0 || na >= n) return 0;
-  for (int b = 0; b < 3; b++) {
-    if (b == a) continue;
-    float dt = r->d[b] * t;
-    float p = r->o[b] + dt;
-    int32_t c = (int32_t)floorf(p);
-    int32_t lo = bc[b] * L, hi = lo + L - 1;
-    if (c < lo) c = lo;
-    if (c > hi) c = hi;
-    if (r->step[b] > 0 && c < w->cell[b]) c = w->cell[b];
-    if (r->step[b] < 0 && c > w->cell[b]) c = w->cell[b];
-    w->cell[b] = c;
+  if (L > 1) {
+    for (int b = 0; b < 3; b++) {
+      if (b == a) continue;
+      float dt = r->d[b] * t;
+      float p = r->o[b] + dt;
+      int32_t c = (int32_t)floorf(p);
+      int32_t lo = bc[b] * L, hi = lo + L - 1;
+      if (hi > n - 1) hi = n - 1;
+      if (c < lo) c = lo;
+      if (c > hi) c = hi;
+      if (r->step[b] > 0 && c < w->cell[b]) c = w->cell[b];
+      if (r->step[b] < 0 && c > w->cell[b]) c = w->cell[b];
+      w->cell[b] = c;
+    }
   }
   w->cell[a] = na;
   w->t = t;
   w->face = entry_face(a, r->step[a]);
-  for (int k = 0; k < 3; k++) tn[k] = next_t(r, k, w->cell[k]);
   return 1;
 }
 
+static int same3(const int32_t a[3], const int32_t b[3]) {
+  return a[0] == b[0] && a[1] == b[1] && a[2] == b[2];
+}
+
 /* Walk from state w until a solid voxel is entered (returns 1), the ray
- * leaves the scene or the entry t reaches tmax (returns 0). */
+ * leaves the scene or the entry t reaches tmax (returns 0).  Each step picks
+ * the coarsest empty level holding the current cell: 128^3 block, 32^3
+ * block, brick, else the voxel itself. */
 static int walk(const mo_scene *s, const ray_t *r, walk_state *w, float tmax, uint16_t *material,
                 mo_counters *cnt) {
   const int32_t n = s->n;
-  float tn[3];
-  for (int a = 0; a < 3; a++) tn[a] = next_t(r, a, w->cell[a]);
-  int32_t cur_node[3] = {-1, -1, -1}, cur_brick[3] = {-1, -1, -1};
+  const int two = s->kind == MO_SCENE_TWO_LEVEL_U16;
+  const int32_t n128 = ceil_div(n, 128), n32 = ceil_div(n, 32);
+  int32_t c128[3] = {-1, -1, -1}, c32[3] = {-1, -1, -1}, cb[3] = {-1, -1, -1};
+  int e128 = 0, e32 = 0;
   uint32_t node = MO_EMPTY_SLOT, slot = MO_EMPTY_SLOT;
-  const uint64_t *mask = NULL;
+  uint64_t groups = 0;
   for (;;) {
     if (!(w->t < tmax)) return 0;
-    int solid;
+    cnt->n_iter++;
+    int32_t L = 1;
     if (s->kind == MO_SCENE_DENSE_U8) {
       cnt->n_vox++;
       uint8_t v = s->dense[lin3(w->cell[0], w->cell[1], w->cell[2], n)];
-      solid = v != 0;
-      if (solid) *material = v;
+      if (v) { *material = v; return 1; }
     } else {
-      if (s->kind == MO_SCENE_TWO_LEVEL_U16) {
-        int32_t nc[3] = {w->cell[0] >> 7, w->cell[1] >> 7, w->cell[2] >> 7};
-        if (nc[0] != cur_node[0] || nc[1] != cur_node[1] || nc[2] != cur_node[2]) {
-          memcpy(cur_node, nc, sizeof(nc));
-          cur_brick[0] = cur_brick[1] = cur_brick[2] = -1;
+      int32_t k[3] = {w->cell[0] >> 7, w->cell[1] >> 7, w->cell[2] >> 7};
+      if (!same3(k, c128)) {
+        memcpy(c128, k, sizeof(k));
+        c32[0] = c32[1] = c32[2] = -1;
+        cb[0] = cb[1] = cb[2] = -1;
+        if (two) {
           cnt->n_top++;
-          node = s->top[lin3(nc[0], nc[1], nc[2], n >> 7)];
-        }
-        if (node == MO_EMPTY_SLOT) {
-          if (!skip_block(r, n, 128, nc, w, tn)) return 0;
-          continue;
-        }
-      }
-      int32_t bc[3] = {w->cell[0] >> 3, w->cell[1] >> 3, w->cell[2] >> 3};
-      if (bc[0] != cur_brick[0] || bc[1] != cur_brick[1] || bc[2] != cur_brick[2]) {
-        memcpy(cur_brick, bc, sizeof(bc));
-        if (s->kind == MO_SCENE_TWO_LEVEL_U16) {
-          cnt->n_node++;
-          slot = s->slots[(int64_t)node * 4096 + lin3(bc[0] & 15, bc[1] & 15, bc[2] & 15, 16)];
+          node = s->top[lin3(k[0], k[1], k[2], n128)];
+          e128 = node == MO_EMPTY_SLOT;
+          groups = e128 ? 0 : s->groups[node];
         } else {
-          cnt->n_brickslot++;
-          slot = s->slots[lin3(bc[0], bc[1], bc[2], n >> 3)];
-        }
-        if (slot != MO_EMPTY_SLOT) {
-          cnt->n_brickmask++;
-          mask = s->masks + (int64_t)slot * 8;
+          cnt->n_coarse++;
+          e128 = !get_bit(s->occ128, lin3(k[0], k[1], k[2], n128));
         }
       }
-      if (slot == MO_EMPTY_SLOT) {
-        if (!skip_block(r, n, 8, bc, w, tn)) return 0;
-        continue;
+      if (e128) {
+        L = 128;
+      } else {
+        int32_t g[3] = {w->cell[0] >> 5, w->cell[1] >> 5, w->cell[2] >> 5};
+        if (!same3(g, c32)) {
+          memcpy(c32, g, sizeof(g));
+          cb[0] = cb[1] = cb[2] = -1;
+          if (two) {
+            e32 = !((groups >> ((g[0] & 3) + 4 * (g[1] & 3) + 16 * (g[2] & 3))) & 1u);
+          } else {
+            cnt->n_coarse++;
+            e32 = !get_bit(s->occ32, lin3(g[0], g[1], g[2], n32));
+          }
+        }
+        if (e32) {
+          L = 32;
+        } else {
+          int32_t bc[3] = {w->cell[0] >> 3, w->cell[1] >> 3, w->cell[2] >> 3};
+          if (!same3(bc, cb)) {
+            memcpy(cb, bc, sizeof(bc));
+            if (two) {
+              cnt->n_node++;
+              slot = s->slots[(int64_t)node * 4096 + lin3(bc[0] & 15, bc[1] & 15, bc[2] & 15, 16)];
+            } else {
+              cnt->n_brickslot++;
+              slot = s->slots[lin3(bc[0], bc[1], bc[2], n >> 3)];
+            }
+            if (slot != MO_EMPTY_SLOT) cnt->n_brickmask++;
+          }
+          if (slot == MO_EMPTY_SLOT) {
+            L = 8;
+          } else {
+            int32_t lx = w->cell[0] & 7, ly = w->cell[1] & 7, lz = w->cell[2] & 7;
+            const uint64_t *mask = s->masks + (int64_t)slot * 8;
+            if ((mask[lz] >> (lx + 8 * ly)) & 1u) {
+              *material = s->voxels[(int64_t)slot * 512 + lx + 8 * ly + 64 * lz];
+              return 1;
+            }
+          }
+        }
       }
-      int32_t lx = w->cell[0] & 7, ly = w->cell[1] & 7, lz = w->cell[2] & 7;
-      solid = (int)((mask[lz] >> (lx + 8 * ly)) & 1u);
-      if (solid) *material = s->voxels[(int64_t)slot * 512 + lx + 8 * ly + 64 * lz];
     }
-    if (solid) return 1;
-    int a = argmin3(tn);
-    w->t = tn[a];
-    w->cell[a] += r->step[a];
-    if (w->cell[a] < 0 || w->cell[a] >= n) return 0;
-    w->face = entry_face(a, r->step[a]);
-    tn[a] = next_t(r, a, w->cell[a]);
+    if (!advance(r, n, L, w)) return 0;
+  }
+}
+
+void mo_build_coarse(const mo_scene *s, int64_t n_nodes, uint32_t *occ32, uint32_t *occ128,
+                     uint64_t *groups) {
+  const int32_t n = s->n, nb = n / 8;
+  if (s->kind == MO_SCENE_BRICKMAP_U16) {
+    const int32_t n32 = ceil_div(n, 32), n128 = ceil_div(n, 128);
+    memset(occ128, 0, sizeof(uint32_t) * (size_t)(((int64_t)n128 * n128 * n128 + 31) / 32));
+    memset(occ32, 0, sizeof(uint32_t) * (size_t)(((int64_t)n32 * n32 * n32 + 31) / 32));
+    for (int32_t bz = 0; bz < nb; bz++)
+      for (int32_t by = 0; by < nb; by++)
+        for (int32_t bx = 0; bx < nb; bx++) {
+          if (s->slots[lin3(bx, by, bz, nb)] == MO_EMPTY_SLOT) continue;
+          int64_t i32 = lin3(bx >> 2, by >> 2, bz >> 2, n32);
+          int64_t i128 = lin3(bx >> 4, by >> 4, bz >> 4, n128);
+          occ32[i32 >> 5] |= 1u << (i32 & 31);
+          occ128[i128 >> 5] |= 1u << (i128 & 31);
+        }
+  } else if (s->kind == MO_SCENE_TWO_LEVEL_U16) {
+    for (int64_t node = 0; node < n_nodes; node++) {
+      uint64_t g = 0;
+      for (int32_t l = 0; l < 4096; l++) {
+        if (s->slots[node * 4096 + l] == MO_EMPTY_SLOT) continue;
+        int32_t bx = l & 15, by = (l >> 4) & 15, bz = l >> 8;
+        g |= 1ull << ((bx >> 2) + 4 * (by >> 2) + 16 * (bz >> 2));
+      }
+      groups[node] = g;
+    }
   }
 }
 
@@ -810,6 +864,8 @@ void mo_raymarch_mt(const mo_scene *s, const mo_camera *cam, uint32_t flags, con
     cnt->rays += jobs[i].cnt.rays;
     cnt->shadow_rays += jobs[i].cnt.shadow_rays;
     cnt->ao_rays += jobs[i].cnt.ao_rays;
+    cnt->n_coarse += jobs[i].cnt.n_coarse;
+    cnt->n_iter += jobs[i].cnt.n_iter;
   }
   free(jobs);
   free(th);

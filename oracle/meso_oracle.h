@@ -142,12 +142,27 @@ typedef struct {
   const uint64_t *masks;
   const uint16_t *voxels;
   const uint32_t *top;       /* two-level: (n/128)^3 node indices */
+  /* empty-space levels above the bricks (see mo_build_coarse) */
+  const uint32_t *occ32;     /* brick map: ceil(n/32)^3 bits */
+  const uint32_t *occ128;    /* brick map: ceil(n/128)^3 bits */
+  const uint64_t *groups;    /* two-level: per node, bit of each 4^3-brick group */
 } mo_scene;
+
+/* Empty-space levels.  Brick map: occ32 / occ128 hold one bit per 32^3 /
+ * 128^3 voxel block (x fastest, word = bit >> 5), set when any non-empty
+ * brick lies in the block.  Two-level: the top node table is the 128 level,
+ * and groups[node] bit gx + 4 gy + 16 gz (g = (brick & 15) >> 2) is the 32
+ * level.  Arrays: occ32 ceil(ceil(n/32)^3 / 32) words, occ128 likewise,
+ * groups n_nodes words. */
+void mo_build_coarse(const mo_scene *s, int64_t n_nodes, uint32_t *occ32, uint32_t *occ128,
+                     uint64_t *groups);
 
 /* step counters for the algorithmic byte model */
 typedef struct {
   int64_t n_vox, n_top, n_node, n_brickslot, n_brickmask, hits;
   int64_t rays, shadow_rays, ao_rays;
+  int64_t n_coarse;  /* occ32 / occ128 words read (brick map) */
+  int64_t n_iter;    /* traversal steps at any level */
 } mo_counters;
 
 typedef struct {

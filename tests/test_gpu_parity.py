@@ -67,7 +67,7 @@ def test_spheres_raymarch_bitexact(ctx, flags):
     _assert_rays_equal(gpu, ref, "config-1 dense")
     cnt = tg.counts()
     rc = ref[4].as_dict()
-    for k in ("n_vox", "hits", "rays", "shadow_rays"):
+    for k in ("n_vox", "n_iter", "hits", "rays", "shadow_rays"):
         assert cnt[k] == rc[k], (k, cnt[k], rc[k])
     sc.destroy()
 
@@ -99,7 +99,7 @@ def test_heightfield_raymarch_full_frame_bitexact(ctx):
     _assert_rays_equal(gpu, ref, "config-2 brick map")
     cnt = tg.counts()
     rc = ref[4].as_dict()
-    for k in ("n_brickslot", "n_brickmask", "hits", "rays", "shadow_rays"):
+    for k in ("n_brickslot", "n_brickmask", "n_coarse", "n_iter", "hits", "rays", "shadow_rays"):
         assert cnt[k] == rc[k], (k, cnt[k], rc[k])
     # FP64 entry-plane t of the chosen voxel within 1e-5 relative
     idx, face, t, _ = gpu
@@ -160,3 +160,66 @@ def test_mesh_batched_matches_oracle(ctx, mode):
     for c in range(0, n, 17):
         ref = O.mesh_chunk(ref_vox[c], mode == L.MESH_GREEDY)
         assert np.array_equal(quads[prefix[c]:prefix[c] + counts[c]], ref), c
+
+
+def test_two_level_caves_generation_and_raymarch_with_ao(ctx):
+    n, seed = 256, 5
+    sc = ctx.scene_generate(L.GEN_CAVES, L.SCENE_TWO_LEVEL_U16, n, seed)
+    a = sc.arrays()
+    slots, masks, vox = O.build_brickmap(O.GEN_CAVES, n, seed)
+    top, ns, m2, v2 = O.two_level_from_bricks(n, slots, masks, vox)
+    assert np.array_equal(a["top"], top)
+    assert np.array_equal(a["slots"], ns)
+    assert np.array_equal(a["masks"], m2) and np.array_equal(a["voxels"], v2)
+    cp = W.caves_camera(n, 320, 180)
+    cam, ocam = _cam_both(cp, 320, 180)
+    sh = R.shading(W.LIGHT, ao_seed=11, ao_samples=4, ao_radius=16.0)
+    tg = R.RayTargets(ctx, 320, 180, index_bytes=8, counters=True)
+    flags = L.RAY_SHADOW | L.RAY_AO
+    R.raymarch(ctx, sc, cam, flags | L.RAY_COUNT, sh, tg)
+    ctx.sync()
+    osc = O.two_level_scene(n, top, ns, m2, v2)
+    ref = O.raymarch(osc, ocam, flags, O.shading(W.LIGHT, 11, 4, 16.0), threads=8)
+    _assert_rays_equal(tg.results(), ref, "two-level caves + AO")
+    cnt, rc = tg.counts(), ref[4].as_dict()
+    for k in ("n_top", "n_node", "n_brickmask", "hits", "rays", "shadow_rays", "ao_rays", "n_iter"):
+        assert cnt[k] == rc[k], (k, cnt[k], rc[k])
+    assert (tg.results()[0] >= 0).mean() > 0.3
+    sc.destroy()
+
+
+def test_brickmap_from_host_arrays_odd_edge(ctx):
+    # edge 200 is not a multiple of 32 or 128: partial coarse blocks
+    n = 200
+    slots, masks, vox = O.build_brickmap(O.GEN_TERRAIN_CAVES, n, 77)
+    sc = ctx.scene_brickmap(n, slots, masks, vox)
+    cp = dict(eye=[-50.0, 180.0, -40.0], target=[100.0, 30.0, 100.0], up=[0.0, 1.0, 0.0], vfov=60.0)
+    cam, ocam = _cam_both(cp, 257, 143)
+    tg = R.RayTargets(ctx, 257, 143, counters=True)
+    R.raymarch(ctx, sc, cam, L.RAY_SHADOW | L.RAY_COUNT, R.shading(W.LIGHT), tg)
+    ctx.sync()
+    ref = O.raymarch(O.brickmap_scene(n, slots, masks, vox), ocam, O.RAY_SHADOW,
+                     O.shading(W.LIGHT), threads=8)
+    _assert_rays_equal(tg.results(), ref, "host-array brick map, edge 200")
+    cnt, rc = tg.counts(), ref[4].as_dict()
+    for k in ("n_brickslot", "n_brickmask", "n_coarse", "n_iter", "hits", "shadow_rays"):
+        assert cnt[k] == rc[k], (k, cnt[k], rc[k])
+    sc.destroy()
+
+
+def test_two_level_from_host_arrays(ctx):
+    n = 256
+    g = np.zeros((n, n, n), np.uint16)
+    g[10:60, 140:200, 30:90] = 3
+    g[200:230, 5:40, 150:250] = 4
+    slots, masks, vox = O.brickmap_from_dense(g)
+    top, ns, m2, v2 = O.two_level_from_bricks(n, slots, masks, vox)
+    sc = ctx.scene_two_level(n, top, ns, m2, v2)
+    cp = dict(eye=[-100.0, 300.0, -80.0], target=[128.0, 100.0, 128.0], up=[0.0, 1.0, 0.0], vfov=50.0)
+    cam, ocam = _cam_both(cp, 200, 150)
+    tg = R.RayTargets(ctx, 200, 150, index_bytes=8)
+    R.raymarch(ctx, sc, cam, L.RAY_SHADOW, R.shading(W.LIGHT), tg)
+    ctx.sync()
+    ref = O.raymarch(O.two_level_scene(n, top, ns, m2, v2), ocam, O.RAY_SHADOW, O.shading(W.LIGHT))
+    _assert_rays_equal(tg.results(), ref, "host-array two-level")
+    sc.destroy()
