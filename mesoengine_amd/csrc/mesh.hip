@@ -1,30 +1,42 @@
 // Culled-face and greedy meshing of 32^3 chunks on gfx950.
 //
-// One workgroup meshes one chunk:
-//  1. Row masks.  Wave64 ballots turn pairs of 32-voxel rows into 32-bit
-//     masks: occupancy O[z][y] (bits over x) and material equality with the
-//     +x, +y and +z neighbour (Ex, Ey, Ez).  A ballot transpose from LDS gives
-//     the same masks with bits over y (OT, EyT, EzT) for the X-facing planes.
-//  2. One lane per (face, slice) plane (6 x 32 = 192 lanes) derives its 32
-//     face rows from those masks and merges them greedily with bit
-//     operations: width from the run of set bits and of equal neighbours,
-//     height by testing whole rows.  Quads are counted first, the workgroup
-//     scans the 192 counts, and the lanes run again to write each quad at its
-//     canonical position (face, slice, v, u).
-//  3. Batched mode: chunks are taken from an atomic ticket, and each chunk
-//     finds its output offset by an ordered decoupled look-back over the
-//     chunks ticketed before it, so one persistent launch writes one compact
-//     quad buffer plus per-chunk counts and exclusive prefixes.
+// One workgroup (256 lanes) meshes one chunk:
+//  1. Row masks.  Each lane loads 8 voxels of a 32-voxel row with one vector
+//     load, plus the same 8 voxels of the +y row and of the +z row, and forms
+//     8-bit masks in registers: occupancy and material equality with the +x,
+//     +y and +z neighbours.  Four lanes hold a row; two xor-shuffles OR their
+//     pieces into 32-bit row masks O[z][y], Ex, Ey, Ez (bits over x) in LDS.
+//     A ballot transpose gives the masks with bits over y (OT, EyT, EzT) for
+//     the X-facing planes.
+//  2. One lane per (face, slice) plane (6 x 32 = 192 lanes) meshes its 32x32
+//     plane with a row sweep in bit arithmetic.  F[v] are the exposed faces of
+//     row v, EH[v] / EV[v] the material equality along u / v.  The quads that
+//     reach row v form disjoint segments (mask `open`, start bits `starts`);
+//     a segment continues into row v+1 iff it lies inside
+//     G = F[v+1] & EV[v] (all its faces exist there with the same material),
+//     tested for all segments at once with a segmented prefix-OR of the
+//     failing bits.  The cells of row v+1 not covered by continuing quads
+//     split into maximal equal-material runs, each one a new quad.  This is
+//     the canonical greedy merge (scan row-major, width first, then height):
+//     a quad's height never depends on cells claimed by other quads, because
+//     quads created before it never share its columns below its first row.
+//     Counting is popcounts of the new-run starts; emitting keeps one packed
+//     record (first row, canonical index) per open quad and writes each quad
+//     when its segment stops.
+//  3. Batched mode: chunks come from an atomic ticket, and each chunk finds
+//     its output offset by an ordered decoupled look-back over the chunks
+//     ticketed before it, so one persistent launch writes one compact quad
+//     buffer plus per-chunk counts and exclusive prefixes.
 #include "meso_internal.h"
 
 namespace meso {
 
-constexpr int kPlanes = 192;    // 6 faces x 32 slices
+constexpr int kPlanes = 192;  // 6 faces x 32 slices
 constexpr int kMeshBlock = 256;
 
 struct ChunkAddr {
   const void *base;
-  uint64_t off;   // element offset of voxel (0,0,0)
+  uint64_t off;     // element offset of voxel (0,0,0)
   uint64_t sy, sz;  // element strides of y and z
 };
 
@@ -46,97 +58,171 @@ __device__ __forceinline__ ChunkAddr chunk_addr(const meso_chunk_source &src, ui
   return a;
 }
 
+// 8 consecutive voxels along x as two (u8) or four (u16) packed words.
 template <int ES>
-__device__ __forceinline__ uint32_t vox(const ChunkAddr &a, int x, int y, int z) {
-  const uint64_t i = a.off + (uint64_t)x + (uint64_t)y * a.sy + (uint64_t)z * a.sz;
-  if constexpr (ES == 1)
-    return ((const uint8_t *)a.base)[i];
-  else
-    return ((const uint16_t *)a.base)[i];
+struct Vox8 {
+  uint32_t w[ES * 2];
+  __device__ __forceinline__ uint32_t get(int j) const {
+    if constexpr (ES == 1)
+      return (w[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+    else
+      return (w[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+  }
+};
+
+template <int ES>
+__device__ __forceinline__ Vox8<ES> load8(const ChunkAddr &a, int x0, int y, int z) {
+  Vox8<ES> r;
+  const uint64_t i = a.off + (uint64_t)x0 + (uint64_t)y * a.sy + (uint64_t)z * a.sz;
+  if constexpr (ES == 1) {
+    const uint2 v = *(const uint2 *)((const uint8_t *)a.base + i);
+    r.w[0] = v.x;
+    r.w[1] = v.y;
+  } else {
+    const uint4 v = *(const uint4 *)((const uint16_t *)a.base + i);
+    r.w[0] = v.x;
+    r.w[1] = v.y;
+    r.w[2] = v.z;
+    r.w[3] = v.w;
+  }
+  return r;
 }
 
+template <int ES>
+__device__ __forceinline__ Vox8<ES> zero8() {
+  Vox8<ES> r;
+#pragma unroll
+  for (int k = 0; k < ES * 2; k++) r.w[k] = 0;
+  return r;
+}
+
+// flat LDS mask planes: O, Ex, Ey, Ez ([z][y], bits over x, row stride 33 so
+// that the 32 lanes reading row v of 32 different z planes hit 32 different
+// banks), OT, EyT, EzT ([z][x], bits over y, stride 32)
+constexpr int kZY = 33;
+enum : int {
+  kO = 0,
+  kEx = 32 * kZY,
+  kEy = 64 * kZY,
+  kEz = 96 * kZY,
+  kOT = 128 * kZY,
+  kEyT = kOT + 1024,
+  kEzT = kOT + 2048,
+  kMaskWords = kOT + 3072
+};
+
 struct MeshLds {
-  uint32_t O[32][32], Ex[32][32], Ey[32][32], Ez[32][32];  // [z][y], bits over x
-  uint32_t OT[32][32], EyT[32][32], EzT[32][32];            // [z][x], bits over y
-  uint32_t R[32][kPlanes];                                   // working face rows [v][plane]
+  uint32_t M[kMaskWords];
+  uint32_t rec[32][kPlanes];  // open quad records [start u][plane]: first row | index << 5
   uint32_t scan[kMeshBlock / 64];
   uint32_t chunk;
   uint32_t base_lo, base_hi;
 };
 
-// Step 1: masks from global memory (each voxel and its +x/+y/+z neighbours).
+// OR of the 8-bit pieces held by the 4 lanes of a row (lanes 4r .. 4r+3)
+__device__ __forceinline__ uint32_t row_or(uint32_t piece) {
+  piece |= __shfl_xor(piece, 1, 64);
+  piece |= __shfl_xor(piece, 2, 64);
+  return piece;
+}
+
+// Step 1: masks.  Wave w handles blocks of 16 rows (rows r = y + 32 z).
 template <int ES>
 __device__ __forceinline__ void build_masks(MeshLds &L, const ChunkAddr &a) {
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   constexpr uint32_t nwaves = kMeshBlock / 64;
-  const int x = (int)(lane & 31u);
-  // 512 row pairs (y even, z)
-  for (uint32_t p = wave; p < 512u; p += nwaves) {
-    const int z = (int)(p >> 4);
-    const int y = (int)((p & 15u) * 2u + (lane >> 5));
-    const uint32_t c = vox<ES>(a, x, y, z);
-    const uint32_t cx = x < 31 ? vox<ES>(a, x + 1, y, z) : 0u;
-    const uint32_t cy = y < 31 ? vox<ES>(a, x, y + 1, z) : 0u;
-    const uint32_t cz = z < 31 ? vox<ES>(a, x, y, z + 1) : 0u;
-    const uint64_t bo = __ballot(c != 0u);
-    const uint64_t bx = __ballot(x < 31 && c == cx);
-    const uint64_t by = __ballot(y < 31 && c == cy);
-    const uint64_t bz = __ballot(z < 31 && c == cz);
-    if (lane == 0) {
-      const int y0 = (int)((p & 15u) * 2u);
-      L.O[z][y0] = (uint32_t)bo;
-      L.O[z][y0 + 1] = (uint32_t)(bo >> 32);
-      L.Ex[z][y0] = (uint32_t)bx;
-      L.Ex[z][y0 + 1] = (uint32_t)(bx >> 32);
-      L.Ey[z][y0] = (uint32_t)by;
-      L.Ey[z][y0 + 1] = (uint32_t)(by >> 32);
-      L.Ez[z][y0] = (uint32_t)bz;
-      L.Ez[z][y0 + 1] = (uint32_t)(bz >> 32);
+  const int k = (int)(lane & 3u);
+#pragma unroll 2
+  for (uint32_t blk = wave; blk < 64u; blk += nwaves) {
+    const uint32_t r = blk * 16u + (lane >> 2);
+    const int y = (int)(r & 31u), z = (int)(r >> 5);
+    const Vox8<ES> c = load8<ES>(a, 8 * k, y, z);
+    const Vox8<ES> cy = y < 31 ? load8<ES>(a, 8 * k, y + 1, z) : zero8<ES>();
+    const Vox8<ES> cz = z < 31 ? load8<ES>(a, 8 * k, y, z + 1) : zero8<ES>();
+    const uint32_t next0 = __shfl_down(c.w[0], 1, 64);  // first voxels of x + 8
+    uint32_t o = 0, ex = 0, ey = 0, ez = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const uint32_t v = c.get(j);
+      o |= (uint32_t)(v != 0u) << j;
+      const uint32_t vx = j < 7 ? c.get(j + 1) : (ES == 1 ? next0 & 0xFFu : next0 & 0xFFFFu);
+      ex |= (uint32_t)(v == vx && (j < 7 || k < 3)) << j;
+      ey |= (uint32_t)(y < 31 && v == cy.get(j)) << j;
+      ez |= (uint32_t)(z < 31 && v == cz.get(j)) << j;
+    }
+    const int sh = 8 * k;
+    o = row_or(o << sh);
+    ex = row_or(ex << sh);
+    ey = row_or(ey << sh);
+    ez = row_or(ez << sh);
+    if (k == 0) {
+      const int zy = z * kZY + y;
+      L.M[kO + zy] = o;
+      L.M[kEx + zy] = ex;
+      L.M[kEy + zy] = ey;
+      L.M[kEz + zy] = ez;
     }
   }
   __syncthreads();
-  // ballot transpose: lane = y + 32 * (z & 1) holds row [z][y]; ballot of bit x
-  // over the lanes is column x (bits over y) of planes z and z+1.
+  // ballot transpose: lane = y + 32 * (z & 1) holds row [z][y]; the ballot of
+  // bit x over the lanes is column x (bits over y) of planes z and z+1.
   for (uint32_t zp = wave; zp < 16u; zp += nwaves) {
-    const int z = (int)(zp * 2u + (lane >> 5));
-    const int y = (int)(lane & 31u);
-    const uint32_t o = L.O[z][y], ey = L.Ey[z][y], ez = L.Ez[z][y];
-    for (int xb = 0; xb < 32; xb++) {
+    const uint32_t z = zp * 2u + (lane >> 5);
+    const uint32_t y = lane & 31u;
+    const uint32_t o = L.M[kO + z * kZY + y], ey = L.M[kEy + z * kZY + y],
+                   ez = L.M[kEz + z * kZY + y];
+    for (uint32_t xb = 0; xb < 32u; xb++) {
       const uint64_t t0 = __ballot((o >> xb) & 1u);
       const uint64_t t1 = __ballot((ey >> xb) & 1u);
       const uint64_t t2 = __ballot((ez >> xb) & 1u);
       if (lane == 0) {
-        const int z0 = (int)(zp * 2u);
-        L.OT[z0][xb] = (uint32_t)t0;
-        L.OT[z0 + 1][xb] = (uint32_t)(t0 >> 32);
-        L.EyT[z0][xb] = (uint32_t)t1;
-        L.EyT[z0 + 1][xb] = (uint32_t)(t1 >> 32);
-        L.EzT[z0][xb] = (uint32_t)t2;
-        L.EzT[z0 + 1][xb] = (uint32_t)(t2 >> 32);
+        const uint32_t z0 = zp * 2u;
+        L.M[kOT + z0 * 32u + xb] = (uint32_t)t0;
+        L.M[kOT + (z0 + 1u) * 32u + xb] = (uint32_t)(t0 >> 32);
+        L.M[kEyT + z0 * 32u + xb] = (uint32_t)t1;
+        L.M[kEyT + (z0 + 1u) * 32u + xb] = (uint32_t)(t1 >> 32);
+        L.M[kEzT + z0 * 32u + xb] = (uint32_t)t2;
+        L.M[kEzT + (z0 + 1u) * 32u + xb] = (uint32_t)(t2 >> 32);
       }
     }
   }
   __syncthreads();
 }
 
-// Face rows of plane (f, s): bit u of row v is set where the face is exposed.
-// z faces: (u,v) = (x,y); y faces: (x,z); x faces: (y,z).
-__device__ __forceinline__ uint32_t face_row(const MeshLds &L, int f, int s, int v) {
-  switch (f) {
-    case 0: return L.O[s][v] & ~(s < 31 ? L.O[s + 1][v] : 0u);
-    case 1: return L.O[s][v] & ~(s > 0 ? L.O[s - 1][v] : 0u);
-    case 2: return L.O[v][s] & ~(s < 31 ? L.O[v][s + 1] : 0u);
-    case 3: return L.O[v][s] & ~(s > 0 ? L.O[v][s - 1] : 0u);
-    case 4: return L.OT[v][s] & ~(s < 31 ? L.OT[v][s + 1] : 0u);
-    default: return L.OT[v][s] & ~(s > 0 ? L.OT[v][s - 1] : 0u);
+// Offsets of plane (f, s) in the mask planes.  z faces: (u,v) = (x,y) and
+// rows are O[s][v]; y faces: (x,z), rows O[v][s]; x faces: (y,z), rows OT[v][s].
+struct PlaneView {
+  int self, nb, eu, ev, stride;  // nb < 0: the neighbouring slice is outside
+};
+
+__device__ __forceinline__ PlaneView plane_view(int f, int s) {
+  PlaneView p;
+  const int ns = (f & 1) ? s - 1 : s + 1;
+  const bool inside = ns >= 0 && ns < 32;
+  if (f < 2) {
+    p.self = kO + s * kZY;
+    p.nb = inside ? kO + ns * kZY : -1;
+    p.eu = kEx + s * kZY;
+    p.ev = kEy + s * kZY;
+    p.stride = 1;
+  } else if (f < 4) {
+    p.self = kO + s;
+    p.nb = inside ? kO + ns : -1;
+    p.eu = kEx + s;
+    p.ev = kEz + s;
+    p.stride = kZY;
+  } else {
+    p.self = kOT + s;
+    p.nb = inside ? kOT + ns : -1;
+    p.eu = kEyT + s;
+    p.ev = kEzT + s;
+    p.stride = 32;
   }
+  return p;
 }
-// equality along u (bit u: cell u == cell u+1) and along v (cell (u,v) == (u,v+1))
-__device__ __forceinline__ uint32_t eq_u(const MeshLds &L, int f, int s, int v) {
-  return f < 2 ? L.Ex[s][v] : (f < 4 ? L.Ex[v][s] : L.EyT[v][s]);
-}
-__device__ __forceinline__ uint32_t eq_v(const MeshLds &L, int f, int s, int v) {
-  return f < 2 ? L.Ey[s][v] : (f < 4 ? L.Ez[v][s] : L.EzT[v][s]);
+
+__device__ __forceinline__ uint32_t face_row(const MeshLds &L, const PlaneView &p, int v) {
+  return L.M[p.self + v * p.stride] & ~(p.nb >= 0 ? L.M[p.nb + v * p.stride] : 0u);
 }
 
 __device__ __forceinline__ uint32_t pack_quad(int f, int s, int u, int v, int w, int h) {
@@ -152,55 +238,116 @@ __device__ __forceinline__ uint32_t pack_quad(int f, int s, int u, int v, int w,
          ((uint32_t)(h - 1) << 20) | ((uint32_t)f << 25);
 }
 
-// Step 2 for one plane.  Returns the quad count; with EMIT writes quad k to
-// out[base + k] while base + k < cap.
-template <bool GREEDY, bool EMIT>
-__device__ __forceinline__ uint32_t mesh_plane(MeshLds &L, int t, uint32_t *out, uint64_t base,
-                                               uint64_t cap) {
-  const int f = t >> 5, s = t & 31;
-  for (int v = 0; v < 32; v++) L.R[v][t] = face_row(L, f, s, v);
+// Bits of the segments that contain a marked bit.  Segments are runs of
+// `cont` bits (a bit continuing the segment of the bit below it) above a start
+// bit; the marks spread up and down through them (prefix-OR by doubling).
+__device__ __forceinline__ uint32_t segment_fill(uint32_t marks, uint32_t cont) {
+  uint32_t up = marks, g = cont;
+#pragma unroll
+  for (int k = 1; k < 32; k <<= 1) {
+    up |= (up << k) & g;
+    g &= g << k;
+  }
+  uint32_t dn = marks, gd = cont >> 1;
+#pragma unroll
+  for (int k = 1; k < 32; k <<= 1) {
+    dn |= (dn >> k) & gd;
+    gd &= gd >> k;
+  }
+  return up | dn;
+}
+
+// Step 2a: quad count of plane t.
+template <bool GREEDY>
+__device__ __forceinline__ uint32_t plane_count(const MeshLds &L, int t) {
+  const PlaneView p = plane_view(t >> 5, t & 31);
   uint32_t count = 0;
+  if (!GREEDY) {
+#pragma unroll 8
+    for (int v = 0; v < 32; v++) count += (uint32_t)__popc(face_row(L, p, v));
+    return count;
+  }
+  uint32_t open = 0, starts = 0, ev_prev = 0;
+#pragma unroll 4
   for (int v = 0; v < 32; v++) {
-    uint32_t row = L.R[v][t];
-    if (!GREEDY && !EMIT) {
-      count += (uint32_t)__popc(row);
-      continue;
+    const uint32_t F = face_row(L, p, v);
+    const uint32_t eh = L.M[p.eu + v * p.stride];
+    if (v > 0) {
+      const uint32_t dead = segment_fill(open & ~(F & ev_prev), open & ~starts) & open;
+      open &= ~dead;
+      starts &= ~dead;
     }
-    while (row) {
-      const int u = __ffs(row) - 1;
-      int w = 1, h = 1;
-      uint32_t wmask = 1u << u;
-      if (GREEDY) {
-        // length of the run of set bits from u; a full row (u = 0, all 32
-        // bits set) leaves no zero in ~row
-        const uint32_t run = ~(row >> u);
-        const int wr = run ? __ffs(run) - 1 : 32;
-        const uint32_t eh = ~(eq_u(L, f, s, v) >> u);
-        const int we = (eh ? __ffs(eh) - 1 : 32) + 1;
-        w = wr < we ? wr : we;
-        wmask = (w >= 32 ? 0xFFFFFFFFu : ((1u << w) - 1u)) << u;
-        const uint32_t emask = w > 1 ? (((1u << (w - 1)) - 1u) << u) : 0u;
-        while (v + h < 32) {
-          const uint32_t r2 = L.R[v + h][t];
-          if ((r2 & wmask) != wmask) break;
-          if ((eq_u(L, f, s, v + h) & emask) != emask) break;
-          if (!((eq_v(L, f, s, v + h - 1) >> u) & 1u)) break;
-          h++;
-        }
-        for (int j = 1; j < h; j++) L.R[v + j][t] &= ~wmask;
-      }
-      row &= ~wmask;
-      if (EMIT) {
-        const uint64_t k = base + count;
-        if (k < cap) out[k] = pack_quad(f, s, u, v, w, h);
-      }
-      count++;
-    }
+    const uint32_t avail = F & ~open;
+    const uint32_t st = avail & ~((avail & eh) << 1);
+    count += (uint32_t)__popc(st);
+    open |= avail;
+    starts |= st;
+    ev_prev = L.M[p.ev + v * p.stride];
   }
   return count;
 }
 
-// exclusive scan of one value per thread over the workgroup; returns the total
+// Close the quads whose start bits are in `bits`: their last row is vend - 1.
+__device__ __forceinline__ void close_quads(const MeshLds &L, int t, int f, int s, uint32_t bits,
+                                            uint32_t open, uint32_t starts, int vend, uint32_t *out,
+                                            uint64_t base, uint64_t cap) {
+  const uint32_t cont = open & ~starts;
+  while (bits) {
+    const int u = __ffs(bits) - 1;
+    bits &= bits - 1;
+    const uint32_t above = u < 31 ? ~(cont >> (u + 1)) : 0u;  // zero where the segment ends
+    const int w = 1 + (u < 31 ? (__ffs(above) - 1) : 0);
+    const uint32_t r = L.rec[u][t];
+    const int v0 = (int)(r & 31u);
+    const uint64_t k = base + (r >> 5);
+    if (k < cap) out[k] = pack_quad(f, s, u, v0, w, vend - v0);
+  }
+}
+
+// Step 2b: write the quads of plane t at out[base + canonical index].
+template <bool GREEDY>
+__device__ __forceinline__ void plane_emit(MeshLds &L, int t, uint32_t *out, uint64_t base,
+                                           uint64_t cap) {
+  const int f = t >> 5, s = t & 31;
+  const PlaneView p = plane_view(f, s);
+  uint32_t count = 0;
+  if (!GREEDY) {
+    for (int v = 0; v < 32; v++) {
+      uint32_t row = face_row(L, p, v);
+      while (row) {
+        const int u = __ffs(row) - 1;
+        row &= row - 1;
+        const uint64_t k = base + count++;
+        if (k < cap) out[k] = pack_quad(f, s, u, v, 1, 1);
+      }
+    }
+    return;
+  }
+  uint32_t open = 0, starts = 0, ev_prev = 0;
+  for (int v = 0; v < 32; v++) {
+    const uint32_t F = face_row(L, p, v);
+    const uint32_t eh = L.M[p.eu + v * p.stride];
+    if (v > 0) {
+      const uint32_t dead = segment_fill(open & ~(F & ev_prev), open & ~starts) & open;
+      close_quads(L, t, f, s, starts & dead, open, starts, v, out, base, cap);
+      open &= ~dead;
+      starts &= ~dead;
+    }
+    const uint32_t avail = F & ~open;
+    uint32_t st = avail & ~((avail & eh) << 1);
+    open |= avail;
+    starts |= st;
+    while (st) {
+      const int u = __ffs(st) - 1;
+      st &= st - 1;
+      L.rec[u][t] = (uint32_t)v | (count++ << 5);
+    }
+    ev_prev = L.M[p.ev + v * p.stride];
+  }
+  close_quads(L, t, f, s, starts, open, starts, 32, out, base, cap);
+}
+
+// exclusive scan of one value per lane over the workgroup; returns the total
 __device__ __forceinline__ uint32_t block_exclusive_scan(MeshLds &L, uint32_t v, uint32_t &excl) {
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   uint32_t inc = v;
@@ -246,7 +393,7 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_batched_kernel(
     if (c >= n_chunks) break;  // uniform: every wave leaves
     const ChunkAddr a = chunk_addr(src, c);
     build_masks<ES>(L, a);
-    const uint32_t mine = t < kPlanes ? mesh_plane<GREEDY, false>(L, t, nullptr, 0, 0) : 0u;
+    const uint32_t mine = t < kPlanes ? plane_count<GREEDY>(L, t) : 0u;
     uint32_t excl = 0;
     const uint32_t chunk_total = block_exclusive_scan(L, mine, excl);
     if (t == 0) {
@@ -286,7 +433,7 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_batched_kernel(
     }
     __syncthreads();
     const uint64_t base = ((uint64_t)L.base_hi << 32) | L.base_lo;
-    if (t < kPlanes) mesh_plane<GREEDY, true>(L, t, quads, base + excl, capacity);
+    if (t < kPlanes) plane_emit<GREEDY>(L, t, quads, base + excl, capacity);
     __syncthreads();
   }
 }
@@ -298,11 +445,11 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_plain_kernel(const void *voxe
   const int t = (int)threadIdx.x;
   const ChunkAddr a{voxels, 0, 32, 1024};
   build_masks<ES>(L, a);
-  const uint32_t mine = t < kPlanes ? mesh_plane<GREEDY, false>(L, t, nullptr, 0, 0) : 0u;
+  const uint32_t mine = t < kPlanes ? plane_count<GREEDY>(L, t) : 0u;
   uint32_t excl = 0;
   const uint32_t chunk_total = block_exclusive_scan(L, mine, excl);
   if (t == 0) *count = chunk_total;
-  if (t < kPlanes) mesh_plane<GREEDY, true>(L, t, quads, excl, capacity);
+  if (t < kPlanes) plane_emit<GREEDY>(L, t, quads, excl, capacity);
 }
 
 template <int ES, bool GREEDY>
@@ -334,6 +481,8 @@ static meso_status check_source(const meso_chunk_source *src) {
   MESO_CHECK(src && src->size >= sizeof(meso_chunk_source), MESO_ERR_ARGUMENT_OUT_OF_RANGE,
              "meso_chunk_source missing or too small");
   MESO_CHECK(src->voxels, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "chunk voxels NULL");
+  MESO_CHECK(((uintptr_t)src->voxels & 15u) == 0, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "chunk voxels must be 16-byte aligned");
   MESO_CHECK(src->elem_bytes == 1 || src->elem_bytes == 2, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
              "elem_bytes must be 1 or 2");
   MESO_CHECK(src->n_chunks >= 1 && src->n_chunks <= 40000, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
@@ -368,23 +517,22 @@ meso_status launch_mesh_batched(meso_context *ctx, const meso_chunk_source *src,
                                         d_work, stream, shape);
 }
 
+template <int ES, bool GREEDY>
+static void launch_plain_t(const void *voxels, uint32_t *d_count, uint32_t *d_quads,
+                           uint32_t capacity, hipStream_t stream) {
+  hipLaunchKernelGGL((mesh_plain_kernel<ES, GREEDY>), dim3(1), dim3(kMeshBlock), 0, stream, voxels,
+                     d_count, d_quads, capacity);
+}
+
 meso_status launch_mesh_plain(const void *voxels, int32_t es, uint32_t mode, uint32_t *d_count,
                               uint32_t *d_quads, uint32_t capacity, hipStream_t stream) {
   const bool g = mode == MESO_MESH_GREEDY;
   if (es == 1) {
-    if (g)
-      hipLaunchKernelGGL((mesh_plain_kernel<1, true>), dim3(1), dim3(kMeshBlock), 0, stream,
-                         voxels, d_count, d_quads, capacity);
-    else
-      hipLaunchKernelGGL((mesh_plain_kernel<1, false>), dim3(1), dim3(kMeshBlock), 0, stream,
-                         voxels, d_count, d_quads, capacity);
+    if (g) launch_plain_t<1, true>(voxels, d_count, d_quads, capacity, stream);
+    else launch_plain_t<1, false>(voxels, d_count, d_quads, capacity, stream);
   } else {
-    if (g)
-      hipLaunchKernelGGL((mesh_plain_kernel<2, true>), dim3(1), dim3(kMeshBlock), 0, stream,
-                         voxels, d_count, d_quads, capacity);
-    else
-      hipLaunchKernelGGL((mesh_plain_kernel<2, false>), dim3(1), dim3(kMeshBlock), 0, stream,
-                         voxels, d_count, d_quads, capacity);
+    if (g) launch_plain_t<2, true>(voxels, d_count, d_quads, capacity, stream);
+    else launch_plain_t<2, false>(voxels, d_count, d_quads, capacity, stream);
   }
   MESO_HIP(hipGetLastError());
   return MESO_OK;
@@ -405,6 +553,8 @@ meso_status meso_mesh_chunk(meso_context *ctx, const void *d_voxels, int32_t ele
                             uint32_t mode, uint32_t *d_count, uint32_t *d_quads, uint32_t capacity,
                             meso_stream stream) {
   MESO_CHECK(ctx && d_voxels && d_count, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  MESO_CHECK(((uintptr_t)d_voxels & 15u) == 0, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "chunk voxels must be 16-byte aligned");
   MESO_CHECK(capacity == 0 || d_quads, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "quads NULL");
   MESO_CHECK(elem_bytes == 1 || elem_bytes == 2, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
              "elem_bytes must be 1 or 2");

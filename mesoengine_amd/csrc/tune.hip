@@ -131,8 +131,8 @@ meso_status meso_graph_destroy(meso_graph *g) {
   return MESO_OK;
 }
 
-// Candidate persistent grids: 1, 2, 4 workgroups per CU, the occupancy bound,
-// and one workgroup per chunk.
+// Candidates: persistent grids of 1..4 workgroups per CU, the occupancy
+// bound, and one workgroup per chunk.
 meso_status meso_autotune_mesh(meso_context *ctx, const meso_chunk_source *src, uint32_t mode,
                                uint32_t *d_counts, uint32_t *d_prefix, uint32_t *d_quads,
                                uint64_t capacity, uint64_t *d_total, void *d_work,
