@@ -177,6 +177,16 @@ meso_status meso_scene_download(meso_scene *scene, int32_t part, void *dst, uint
  * keeps a zero mask.  d_voxels: n_edit * 512 u16. */
 meso_status meso_scene_update_bricks(meso_scene *scene, const uint32_t *d_slots,
                                      const uint16_t *d_voxels, int32_t n_edit, meso_stream stream);
+/* Seeded edits of 32^3 chunks (chunk coordinates d_chunk_xyz[3k], device):
+ * inside a sphere drawn per chunk from lowbias32(seed ^ lowbias32(chunk
+ * index)) every voxel of an allocated brick toggles (solid -> 0, empty -> 7)
+ * and the brick masks follow.  Bricks are neither allocated nor freed.  The
+ * chunk grid has ceil(n/32) chunks per axis (chunk index x + nc*(y + nc*z));
+ * chunks outside it are skipped.
+ * Replaces the per-chunk regeneration path FChunkManage::MultiThreadGenerator
+ * (Runtimes/Voxel/Chunk/ChunkManager.h:160-210) for edited chunks. */
+meso_status meso_scene_edit_chunks(meso_scene *scene, const int32_t *d_chunk_xyz, int32_t n_chunks,
+                                   uint32_t seed, meso_stream stream);
 meso_status meso_scene_destroy(meso_scene *scene);
 
 /* ------------------------------------------------------------------ */
@@ -271,6 +281,16 @@ meso_status meso_mesh_chunks(meso_context *ctx, const meso_chunk_source *src, ui
                              uint32_t *d_counts, uint32_t *d_prefix, uint32_t *d_quads,
                              uint64_t capacity, uint64_t *d_total, void *d_work,
                              meso_stream stream);
+
+/* Batched meshing of 32^3 chunks read straight from a resident brick map or
+ * two-level scene (chunk coordinates d_chunk_xyz[3k] on the device, on the
+ * grid of ceil(n/32) chunks per axis; voxels past the scene edge and chunks
+ * outside the grid mesh as empty).  Outputs and workspace as in
+ * meso_mesh_chunks; quads are chunk-local. */
+meso_status meso_mesh_scene_chunks(meso_context *ctx, meso_scene *scene, const int32_t *d_chunk_xyz,
+                                   int32_t n_chunks, uint32_t mode, uint32_t *d_counts,
+                                   uint32_t *d_prefix, uint32_t *d_quads, uint64_t capacity,
+                                   uint64_t *d_total, void *d_work, meso_stream stream);
 
 /* Captures meso_mesh_chunks (workspace reset + persistent launch) into a
  * hipGraph that replays with the same arguments. */

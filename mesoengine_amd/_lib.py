@@ -123,6 +123,9 @@ _SIGS = {
     "meso_scene_get_info": (C.c_int32, [P, P]),
     "meso_scene_download": (C.c_int32, [P, C.c_int32, P, C.c_uint64]),
     "meso_scene_update_bricks": (C.c_int32, [P, P, P, C.c_int32, P]),
+    "meso_scene_edit_chunks": (C.c_int32, [P, P, C.c_int32, C.c_uint32, P]),
+    "meso_mesh_scene_chunks": (C.c_int32, [P, P, P, C.c_int32, C.c_uint32, P, P, P, C.c_uint64,
+                                           P, P, P]),
     "meso_scene_destroy": (C.c_int32, [P]),
     "meso_raymarch": (C.c_int32, [P, P, P, C.c_uint32, P, P, P]),
     "meso_mesh_chunk": (C.c_int32, [P, P, C.c_int32, C.c_uint32, P, P, C.c_uint32, P]),

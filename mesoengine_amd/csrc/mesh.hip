@@ -96,6 +96,42 @@ __device__ __forceinline__ Vox8<ES> zero8() {
   return r;
 }
 
+// Rows of a 32^3 chunk of a resident brick scene (KIND 2 brick map, 3
+// two-level): the 8 voxels x0..x0+7 of a row are one 16-byte row of a brick.
+template <int KIND>
+struct SceneRows {
+  DevScene s;
+  int32_t ox, oy, oz;  // voxel origin of the chunk
+};
+
+template <int ES, int KIND>
+__device__ __forceinline__ Vox8<2> load8(const SceneRows<KIND> &a, int x0, int y, int z) {
+  Vox8<2> r = zero8<2>();
+  const int32_t gx = a.ox + x0, gy = a.oy + y, gz = a.oz + z;
+  const int32_t nb = a.s.n >> 3;
+  const int32_t bx = gx >> 3, by = gy >> 3, bz = gz >> 3;
+  if (bx >= nb || by >= nb || bz >= nb) return r;
+  uint32_t slot;
+  if constexpr (KIND == 2) {
+    slot = a.s.slots[(uint32_t)bx + ((uint32_t)by + (uint32_t)bz * (uint32_t)nb) * (uint32_t)nb];
+  } else {
+    const uint32_t nt = (uint32_t)a.s.n >> 7;
+    const uint32_t node =
+        a.s.top[((uint32_t)bx >> 4) + (((uint32_t)by >> 4) + ((uint32_t)bz >> 4) * nt) * nt];
+    if (node == 0xFFFFFFFFu) return r;
+    slot = a.s.slots[(uint64_t)node * 4096u + ((uint32_t)bx & 15u) + 16u * ((uint32_t)by & 15u) +
+                     256u * ((uint32_t)bz & 15u)];
+  }
+  if (slot == 0xFFFFFFFFu) return r;
+  const uint4 v = *(const uint4 *)(a.s.voxels + (uint64_t)slot * 512u + 8u * (uint32_t)(gy & 7) +
+                                   64u * (uint32_t)(gz & 7));
+  r.w[0] = v.x;
+  r.w[1] = v.y;
+  r.w[2] = v.z;
+  r.w[3] = v.w;
+  return r;
+}
+
 // flat LDS mask planes: O, Ex, Ey, Ez ([z][y], bits over x, row stride 33 so
 // that the 32 lanes reading row v of 32 different z planes hit 32 different
 // banks), OT, EyT, EzT ([z][x], bits over y, stride 32)
@@ -127,8 +163,8 @@ __device__ __forceinline__ uint32_t row_or(uint32_t piece) {
 }
 
 // Step 1: masks.  Wave w handles blocks of 16 rows (rows r = y + 32 z).
-template <int ES>
-__device__ __forceinline__ void build_masks(MeshLds &L, const ChunkAddr &a) {
+template <int ES, class Rows>
+__device__ __forceinline__ void build_masks(MeshLds &L, const Rows &a) {
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   constexpr uint32_t nwaves = kMeshBlock / 64;
   const int k = (int)(lane & 3u);
@@ -379,20 +415,36 @@ struct MeshWork {
   unsigned long long status[1];  // n_chunks entries
 };
 
-template <int ES, bool GREEDY>
+// chunks of a resident scene: chunk coordinates on the device
+struct SceneSource {
+  DevScene s;
+  const int32_t *xyz;
+  int32_t n_chunks;
+};
+
+// SRC 0: packed or dense-grid chunks (meso_chunk_source); 2 / 3: chunks of a
+// brick map / two-level scene
+template <int ES, bool GREEDY, int SRC>
 __global__ void __launch_bounds__(kMeshBlock) mesh_batched_kernel(
-    meso_chunk_source src, uint32_t *counts, uint32_t *prefix, uint32_t *quads, uint64_t capacity,
-    unsigned long long *total, MeshWork *work) {
+    meso_chunk_source src, SceneSource ss, uint32_t *counts, uint32_t *prefix, uint32_t *quads,
+    uint64_t capacity, unsigned long long *total, MeshWork *work) {
   __shared__ MeshLds L;
   const int t = (int)threadIdx.x;
-  const uint32_t n_chunks = (uint32_t)src.n_chunks;
+  const uint32_t n_chunks = (uint32_t)(SRC == 0 ? src.n_chunks : ss.n_chunks);
   for (;;) {
     if (t == 0) L.chunk = atomicAdd(&work->ticket, 1u);
     __syncthreads();
     const uint32_t c = L.chunk;
     if (c >= n_chunks) break;  // uniform: every wave leaves
-    const ChunkAddr a = chunk_addr(src, c);
-    build_masks<ES>(L, a);
+    if constexpr (SRC == 0) {
+      build_masks<ES>(L, chunk_addr(src, c));
+    } else {
+      const int32_t nc = (ss.s.n + 31) >> 5;  // a partial last chunk counts
+      int32_t cx = ss.xyz[3 * c], cy = ss.xyz[3 * c + 1], cz = ss.xyz[3 * c + 2];
+      if (cx < 0 || cy < 0 || cz < 0 || cx >= nc || cy >= nc || cz >= nc)
+        cx = cy = cz = nc;  // outside the scene: every brick lookup reads empty
+      build_masks<ES>(L, SceneRows<SRC>{ss.s, cx * 32, cy * 32, cz * 32});
+    }
     const uint32_t mine = t < kPlanes ? plane_count<GREEDY>(L, t) : 0u;
     uint32_t excl = 0;
     const uint32_t chunk_total = block_exclusive_scan(L, mine, excl);
@@ -452,25 +504,26 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_plain_kernel(const void *voxe
   if (t < kPlanes) plane_emit<GREEDY>(L, t, quads, excl, capacity);
 }
 
-template <int ES, bool GREEDY>
-static meso_status launch_batched_t(meso_context *ctx, const meso_chunk_source *src,
-                                    uint32_t *d_counts, uint32_t *d_prefix, uint32_t *d_quads,
-                                    uint64_t capacity, uint64_t *d_total, void *d_work,
-                                    hipStream_t stream, const MeshLaunch *shape) {
+template <int ES, bool GREEDY, int SRC>
+static meso_status launch_batched_t(meso_context *ctx, const meso_chunk_source &src,
+                                    const SceneSource &ss, int32_t n_chunks, uint32_t *d_counts,
+                                    uint32_t *d_prefix, uint32_t *d_quads, uint64_t capacity,
+                                    uint64_t *d_total, void *d_work, hipStream_t stream,
+                                    const MeshLaunch *shape) {
   int grid = shape && shape->grid > 0 ? shape->grid : 0;
   if (!grid) {
     int per_cu = 0;
     MESO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, mesh_batched_kernel<ES, GREEDY>, kMeshBlock, 0));
+        &per_cu, mesh_batched_kernel<ES, GREEDY, SRC>, kMeshBlock, 0));
     if (per_cu < 1) per_cu = 1;
     grid = per_cu * ctx->props.multiProcessorCount;
   }
-  if (grid > src->n_chunks) grid = src->n_chunks;
-  meso_status st = launch_zero(d_work, meso_mesh_workspace_bytes(src->n_chunks), stream);
+  if (grid > n_chunks) grid = n_chunks;
+  meso_status st = launch_zero(d_work, meso_mesh_workspace_bytes(n_chunks), stream);
   if (st != MESO_OK) return st;
-  hipLaunchKernelGGL((mesh_batched_kernel<ES, GREEDY>), dim3(grid), dim3(kMeshBlock), 0, stream,
-                     *src, d_counts, d_prefix, d_quads, capacity, (unsigned long long *)d_total,
-                     (MeshWork *)d_work);
+  hipLaunchKernelGGL((mesh_batched_kernel<ES, GREEDY, SRC>), dim3(grid), dim3(kMeshBlock), 0,
+                     stream, src, ss, d_counts, d_prefix, d_quads, capacity,
+                     (unsigned long long *)d_total, (MeshWork *)d_work);
   MESO_HIP(hipGetLastError());
   return MESO_OK;
 }
@@ -506,15 +559,36 @@ meso_status launch_mesh_batched(meso_context *ctx, const meso_chunk_source *src,
                                 uint64_t capacity, uint64_t *d_total, void *d_work,
                                 hipStream_t stream, const MeshLaunch *shape) {
   const bool g = mode == MESO_MESH_GREEDY;
+  const SceneSource none{};
+  const int32_t n = src->n_chunks;
   if (src->elem_bytes == 1)
-    return g ? launch_batched_t<1, true>(ctx, src, d_counts, d_prefix, d_quads, capacity, d_total,
-                                         d_work, stream, shape)
-             : launch_batched_t<1, false>(ctx, src, d_counts, d_prefix, d_quads, capacity,
-                                          d_total, d_work, stream, shape);
-  return g ? launch_batched_t<2, true>(ctx, src, d_counts, d_prefix, d_quads, capacity, d_total,
-                                       d_work, stream, shape)
-           : launch_batched_t<2, false>(ctx, src, d_counts, d_prefix, d_quads, capacity, d_total,
-                                        d_work, stream, shape);
+    return g ? launch_batched_t<1, true, 0>(ctx, *src, none, n, d_counts, d_prefix, d_quads,
+                                            capacity, d_total, d_work, stream, shape)
+             : launch_batched_t<1, false, 0>(ctx, *src, none, n, d_counts, d_prefix, d_quads,
+                                             capacity, d_total, d_work, stream, shape);
+  return g ? launch_batched_t<2, true, 0>(ctx, *src, none, n, d_counts, d_prefix, d_quads, capacity,
+                                          d_total, d_work, stream, shape)
+           : launch_batched_t<2, false, 0>(ctx, *src, none, n, d_counts, d_prefix, d_quads,
+                                           capacity, d_total, d_work, stream, shape);
+}
+
+static meso_status launch_mesh_scene(meso_context *ctx, const meso_scene *scene,
+                                     const int32_t *xyz, int32_t n, uint32_t mode,
+                                     uint32_t *d_counts, uint32_t *d_prefix, uint32_t *d_quads,
+                                     uint64_t capacity, uint64_t *d_total, void *d_work,
+                                     hipStream_t stream) {
+  const meso_chunk_source none{};
+  const SceneSource ss{scene->dev(), xyz, n};
+  const bool g = mode == MESO_MESH_GREEDY;
+  if (scene->kind == MESO_SCENE_BRICKMAP_U16)
+    return g ? launch_batched_t<2, true, 2>(ctx, none, ss, n, d_counts, d_prefix, d_quads,
+                                            capacity, d_total, d_work, stream, nullptr)
+             : launch_batched_t<2, false, 2>(ctx, none, ss, n, d_counts, d_prefix, d_quads,
+                                             capacity, d_total, d_work, stream, nullptr);
+  return g ? launch_batched_t<2, true, 3>(ctx, none, ss, n, d_counts, d_prefix, d_quads, capacity,
+                                          d_total, d_work, stream, nullptr)
+           : launch_batched_t<2, false, 3>(ctx, none, ss, n, d_counts, d_prefix, d_quads,
+                                           capacity, d_total, d_work, stream, nullptr);
 }
 
 template <int ES, bool GREEDY>
@@ -562,6 +636,24 @@ meso_status meso_mesh_chunk(meso_context *ctx, const void *d_voxels, int32_t ele
              "unknown mesh mode %u", mode);
   return launch_mesh_plain(d_voxels, elem_bytes, mode, d_count, d_quads, capacity,
                            stream_of(ctx, stream));
+}
+
+meso_status meso_mesh_scene_chunks(meso_context *ctx, meso_scene *scene, const int32_t *d_chunk_xyz,
+                                   int32_t n_chunks, uint32_t mode, uint32_t *d_counts,
+                                   uint32_t *d_prefix, uint32_t *d_quads, uint64_t capacity,
+                                   uint64_t *d_total, void *d_work, meso_stream stream) {
+  MESO_CHECK(ctx && scene && d_chunk_xyz, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  MESO_CHECK(scene->ctx == ctx, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "scene of another context");
+  MESO_CHECK(scene->kind == MESO_SCENE_BRICKMAP_U16 || scene->kind == MESO_SCENE_TWO_LEVEL_U16,
+             MESO_ERR_NOT_SUPPORTED, "scene meshing needs a brick map or two-level scene");
+  MESO_CHECK(n_chunks >= 1 && n_chunks <= 40000, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "n_chunks %d not in 1..40000", n_chunks);
+  MESO_CHECK(d_counts && d_prefix && d_total && d_work && (capacity == 0 || d_quads),
+             MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL output");
+  MESO_CHECK(mode == MESO_MESH_CULLED || mode == MESO_MESH_GREEDY, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "unknown mesh mode %u", mode);
+  return launch_mesh_scene(ctx, scene, d_chunk_xyz, n_chunks, mode, d_counts, d_prefix, d_quads,
+                           capacity, d_total, d_work, stream_of(ctx, stream));
 }
 
 meso_status meso_mesh_chunks(meso_context *ctx, const meso_chunk_source *src, uint32_t mode,

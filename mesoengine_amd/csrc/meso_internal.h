@@ -131,6 +131,17 @@ struct meso_graph {
 };
 
 namespace meso {
+// A 1D launch of nblocks workgroups spread over a 2D grid, so that
+// gridDim.x * blockDim.x stays below 2^32 work-items; kernels recover the
+// linear block with block_id() and skip ids >= nblocks.
+inline dim3 grid_1d(uint64_t nblocks) {
+  const uint64_t gx = nblocks < 65536u ? (nblocks ? nblocks : 1u) : 65536u;
+  return dim3((uint32_t)gx, (uint32_t)((nblocks + gx - 1) / gx));
+}
+__device__ __forceinline__ uint64_t block_id() {
+  return (uint64_t)blockIdx.y * gridDim.x + blockIdx.x;
+}
+
 // host helpers shared across translation units
 void derive_camera(const meso_camera *c, DevCamera *out);
 void derive_shading(const meso_shading *s, DevShading *out);

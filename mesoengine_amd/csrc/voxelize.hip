@@ -99,19 +99,25 @@ __device__ __forceinline__ bool in_sphere_fp(const int32_t *s, int32_t x, int32_
 __global__ void spheres_dense_kernel(uint8_t *grid, int32_t n, SphereSet set,
                                      unsigned long long *solid) {
   const uint64_t total = (uint64_t)n * n * n;
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint8_t m = 0;
-  if (i < total) {
-    const int32_t x = (int32_t)(i % n), y = (int32_t)((i / n) % n), z = (int32_t)(i / ((uint64_t)n * n));
-    for (int k = 0; k < set.count; k++)
-      if (in_sphere_fp(&set.p[4 * k], x, y, z)) {
-        m = (uint8_t)(1 + k);
-        break;
-      }
-    grid[i] = m;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // grid-stride: every wave runs the same number of iterations
+  const uint64_t rounds = (total + stride - 1) / stride;
+  for (uint64_t it = 0; it < rounds; it++) {
+    const uint64_t i = it * stride + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint8_t m = 0;
+    if (i < total) {
+      const int32_t x = (int32_t)(i % n), y = (int32_t)((i / n) % n),
+                    z = (int32_t)(i / ((uint64_t)n * n));
+      for (int k = 0; k < set.count; k++)
+        if (in_sphere_fp(&set.p[4 * k], x, y, z)) {
+          m = (uint8_t)(1 + k);
+          break;
+        }
+      grid[i] = m;
+    }
+    const unsigned long long b = __ballot(m != 0);
+    if ((threadIdx.x & 63u) == 0 && b) atomicAdd(solid, (unsigned long long)__popcll(b));
   }
-  const unsigned long long b = __ballot(m != 0);
-  if ((threadIdx.x & 63u) == 0 && b) atomicAdd(solid, (unsigned long long)__popcll(b));
 }
 
 // ------------------------------------------------------------------ brick scenes
@@ -157,10 +163,13 @@ __device__ __forceinline__ void brick_of(uint64_t ordered, int32_t nb, bool two_
 __global__ void __launch_bounds__(512) brick_flags_kernel(int32_t gen, int32_t n, uint32_t seed,
                                                           const int32_t *heights, int32_t two_level,
                                                           uint32_t *flags,
-                                                          unsigned long long *solid) {
+                                                          unsigned long long *solid,
+                                                          uint64_t n_bricks) {
+  const uint64_t bid = block_id();
+  if (bid >= n_bricks) return;  // uniform over the workgroup
   const int32_t nb = n / 8;
   int32_t bx, by, bz;
-  brick_of(blockIdx.x, nb, two_level != 0, bx, by, bz);
+  brick_of(bid, nb, two_level != 0, bx, by, bz);
   const int32_t lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
   const uint16_t v = gen_voxel(gen, bx * 8 + lx, by * 8 + ly, bz * 8 + lz, n, seed, heights);
   const unsigned long long b = __ballot(v != 0);
@@ -170,7 +179,7 @@ __global__ void __launch_bounds__(512) brick_flags_kernel(int32_t gen, int32_t n
   if ((threadIdx.x & 63u) == 0 && b) atomicAdd(&cnt, (uint32_t)__popcll(b));
   __syncthreads();
   if (threadIdx.x == 0) {
-    flags[blockIdx.x] = cnt ? 1u : 0u;
+    flags[bid] = cnt ? 1u : 0u;
     if (cnt) atomicAdd(solid, (unsigned long long)cnt);
   }
 }
@@ -181,9 +190,10 @@ __global__ void __launch_bounds__(512) brick_write_kernel(int32_t gen, int32_t n
                                                           const uint32_t *prefix,
                                                           const uint32_t *node_ids,
                                                           uint32_t *slots, uint64_t *masks,
-                                                          uint16_t *voxels) {
+                                                          uint16_t *voxels, uint64_t n_bricks) {
   const int32_t nb = n / 8;
-  const uint64_t ordered = blockIdx.x;
+  const uint64_t ordered = block_id();
+  if (ordered >= n_bricks) return;  // uniform over the workgroup
   if (two_level) {
     // bricks of empty nodes have no slot table
     const uint32_t node = node_ids[ordered / 4096u];
@@ -334,16 +344,70 @@ __global__ void __launch_bounds__(256) sphere_chunk_kernel(const int32_t *chunk_
 __global__ void __launch_bounds__(512) brick_update_kernel(const uint32_t *edit_slots,
                                                            const uint16_t *edit_vox,
                                                            int64_t n_bricks, uint64_t *masks,
-                                                           uint16_t *voxels, uint32_t *bad) {
-  const uint32_t slot = edit_slots[blockIdx.x];
+                                                           uint16_t *voxels, uint32_t *bad,
+                                                           uint64_t n_edit) {
+  const uint64_t e = block_id();
+  if (e >= n_edit) return;
+  const uint32_t slot = edit_slots[e];
   if ((int64_t)slot >= n_bricks) {  // every lane takes the same branch
     if (threadIdx.x == 0) atomicAdd(bad, 1u);
     return;
   }
-  const uint16_t v = edit_vox[(uint64_t)blockIdx.x * 512u + threadIdx.x];
+  const uint16_t v = edit_vox[e * 512u + threadIdx.x];
   const unsigned long long b = __ballot(v != 0);
   voxels[(uint64_t)slot * 512u + threadIdx.x] = v;
   if ((threadIdx.x & 63u) == 0) masks[(uint64_t)slot * 8u + (threadIdx.x >> 6)] = (uint64_t)b;
+}
+
+// ------------------------------------------------------------------ chunk edits
+// slot of brick (bx,by,bz) in a brick map or two-level scene, or kEmptySlot
+__device__ __forceinline__ uint32_t scene_brick_slot(const DevScene &s, int32_t kind, int32_t bx,
+                                                     int32_t by, int32_t bz) {
+  const int32_t nb = s.n >> 3;
+  if (bx < 0 || by < 0 || bz < 0 || bx >= nb || by >= nb || bz >= nb) return kEmptySlot;
+  if (kind == MESO_SCENE_BRICKMAP_U16)
+    return s.slots[(uint32_t)bx + ((uint32_t)by + (uint32_t)bz * (uint32_t)nb) * (uint32_t)nb];
+  const uint32_t nt = (uint32_t)s.n >> 7;
+  const uint32_t node = s.top[((uint32_t)bx >> 4) + (((uint32_t)by >> 4) + ((uint32_t)bz >> 4) * nt) * nt];
+  if (node == kEmptySlot) return kEmptySlot;
+  return s.slots[(uint64_t)node * 4096u + ((uint32_t)bx & 15u) + 16u * ((uint32_t)by & 15u) +
+                 256u * ((uint32_t)bz & 15u)];
+}
+
+// one workgroup of 512 lanes per (chunk, brick of the chunk); lane = voxel
+__global__ void __launch_bounds__(512) edit_chunks_kernel(DevScene s, int32_t kind,
+                                                          const int32_t *xyz, uint32_t seed,
+                                                          uint64_t *masks, uint16_t *voxels,
+                                                          uint64_t n_blocks) {
+  const uint64_t bid = block_id();
+  if (bid >= n_blocks) return;
+  const uint64_t chunk = bid >> 6;
+  const uint32_t b = (uint32_t)bid & 63u;
+  const int32_t nc = (s.n + 31) >> 5;  // a partial last chunk counts
+  const int32_t cx = xyz[3 * chunk], cy = xyz[3 * chunk + 1], cz = xyz[3 * chunk + 2];
+  if (cx < 0 || cy < 0 || cz < 0 || cx >= nc || cy >= nc || cz >= nc) return;
+  const int32_t bx = cx * 4 + (int32_t)(b & 3u), by = cy * 4 + (int32_t)((b >> 2) & 3u),
+                bz = cz * 4 + (int32_t)(b >> 4);
+  const uint32_t slot = scene_brick_slot(s, kind, bx, by, bz);
+  if (slot == kEmptySlot) return;  // uniform over the workgroup
+  const uint32_t h0 = nhash32(seed ^ nhash32((uint32_t)cx + ((uint32_t)cy + (uint32_t)cz * (uint32_t)nc) * (uint32_t)nc));
+  const uint32_t h1 = nhash32(h0), h2 = nhash32(h1), h3 = nhash32(h2);
+  const int64_t sx = (int64_t)cx * 8192 + (h0 & 0x1FFFu), sy = (int64_t)cy * 8192 + (h1 & 0x1FFFu),
+                sz = (int64_t)cz * 8192 + (h2 & 0x1FFFu);
+  const int64_t r = 4 * 256 + (int64_t)(h3 % (8u * 256u));
+  const uint32_t i = threadIdx.x;
+  const int32_t lx = (int32_t)(i & 7u), ly = (int32_t)((i >> 3) & 7u), lz = (int32_t)(i >> 6);
+  const int64_t dx = (int64_t)(bx * 8 + lx) * 256 + 128 - sx;
+  const int64_t dy = (int64_t)(by * 8 + ly) * 256 + 128 - sy;
+  const int64_t dz = (int64_t)(bz * 8 + lz) * 256 + 128 - sz;
+  uint16_t *v = voxels + (uint64_t)slot * 512u + i;
+  uint16_t val = *v;
+  if (dx * dx + dy * dy + dz * dz <= r * r) {
+    val = val ? (uint16_t)0 : (uint16_t)7;
+    *v = val;
+  }
+  const unsigned long long m = __ballot(val != 0);
+  if ((i & 63u) == 0) masks[(uint64_t)slot * 8u + (uint32_t)lz] = (uint64_t)m;
 }
 
 }  // namespace meso
@@ -414,7 +478,7 @@ meso_status meso_scene_generate(meso_context *ctx, const meso_generate_desc *d, 
     unsigned long long *solid = nullptr;
     GEN_HIP(hipMallocAsync((void **)&solid, 8, stream));
     GEN_TRY(launch_zero(solid, 8, stream));
-    hipLaunchKernelGGL(spheres_dense_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0,
+    hipLaunchKernelGGL(spheres_dense_kernel, dim3((uint32_t)std::min<uint64_t>((total + 255) / 256, 1u << 20)), dim3(256), 0,
                        stream, sc->dense, n, set, solid);
     GEN_HIP(hipGetLastError());
     unsigned long long s = 0;
@@ -453,8 +517,8 @@ meso_status meso_scene_generate(meso_context *ctx, const meso_generate_desc *d, 
   GEN_HIP(hipMallocAsync((void **)&prefix, n_brick_cells * 4, stream));
   GEN_HIP(hipMallocAsync((void **)&solid, 8, stream));
   GEN_TRY(launch_zero(solid, 8, stream));
-  hipLaunchKernelGGL(brick_flags_kernel, dim3((uint32_t)n_brick_cells), dim3(512), 0, stream,
-                     d->generator, n, seed, heights, two ? 1 : 0, flags, solid);
+  hipLaunchKernelGGL(brick_flags_kernel, grid_1d(n_brick_cells), dim3(512), 0, stream,
+                     d->generator, n, seed, heights, two ? 1 : 0, flags, solid, n_brick_cells);
   GEN_HIP(hipGetLastError());
   uint64_t n_bricks = 0;
   GEN_TRY(scan_u32(flags, prefix, n_brick_cells, &n_bricks, stream));
@@ -481,9 +545,9 @@ meso_status meso_scene_generate(meso_context *ctx, const meso_generate_desc *d, 
   GEN_HIP(hipMalloc((void **)&sc->masks, std::max<uint64_t>(n_bricks, 1) * 64));
   GEN_HIP(hipMalloc((void **)&sc->voxels, std::max<uint64_t>(n_bricks, 1) * 1024));
   sc->bytes += n_bricks * (64 + 1024);
-  hipLaunchKernelGGL(brick_write_kernel, dim3((uint32_t)n_brick_cells), dim3(512), 0, stream,
+  hipLaunchKernelGGL(brick_write_kernel, grid_1d(n_brick_cells), dim3(512), 0, stream,
                      d->generator, n, seed, heights, two ? 1 : 0, flags, prefix, sc->top, sc->slots,
-                     sc->masks, sc->voxels);
+                     sc->masks, sc->voxels, n_brick_cells);
   GEN_HIP(hipGetLastError());
   unsigned long long s = 0;
   GEN_HIP(hipMemcpyAsync(&s, solid, 8, hipMemcpyDeviceToHost, stream));
@@ -558,7 +622,8 @@ meso_status meso_generate_sphere_chunks(meso_context *ctx, const int32_t *d_chun
                                         int32_t n_chunks, int32_t res, float block_size,
                                         int32_t depth, uint8_t *d_mips, meso_stream stream) {
   MESO_CHECK(ctx && d_chunk_xyz && d_mips, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
-  MESO_CHECK(n_chunks >= 1 && res >= 1 && res <= 32 && depth >= 1 && depth <= 8,
+  MESO_CHECK(n_chunks >= 1 && n_chunks <= (1 << 20) && res >= 1 && res <= 32 && depth >= 1 &&
+                 depth <= 8,
              MESO_ERR_ARGUMENT_OUT_OF_RANGE, "n_chunks %d, res %d (1..32), depth %d (1..8)",
              n_chunks, res, depth);
   hipStream_t s = stream_of(ctx, stream);
@@ -566,6 +631,23 @@ meso_status meso_generate_sphere_chunks(meso_context *ctx, const int32_t *d_chun
   const size_t lds = 2u * (size_t)res * res * res;
   hipLaunchKernelGGL(sphere_chunk_kernel, dim3((uint32_t)n_chunks), dim3(256), lds, s,
                      d_chunk_xyz, res, (double)block_size, depth, d_mips);
+  MESO_HIP(hipGetLastError());
+  return MESO_OK;
+}
+
+meso_status meso_scene_edit_chunks(meso_scene *scene, const int32_t *d_chunk_xyz, int32_t n_chunks,
+                                   uint32_t seed, meso_stream stream) {
+  MESO_CHECK(scene && d_chunk_xyz, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  MESO_CHECK(scene->kind != MESO_SCENE_DENSE_U8, MESO_ERR_NOT_SUPPORTED,
+             "chunk edits need a brick map or two-level scene");
+  MESO_CHECK(n_chunks >= 0 && n_chunks <= (1 << 24), MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "n_chunks %d out of range", n_chunks);
+  if (n_chunks == 0) return MESO_OK;
+  hipStream_t s = stream_of(scene->ctx, stream);
+  MESO_HIP(hipSetDevice(scene->ctx->device));
+  const uint64_t n_blocks = (uint64_t)n_chunks * 64u;
+  hipLaunchKernelGGL(edit_chunks_kernel, grid_1d(n_blocks), dim3(512), 0, s, scene->dev(),
+                     scene->kind, d_chunk_xyz, seed, scene->masks, scene->voxels, n_blocks);
   MESO_HIP(hipGetLastError());
   return MESO_OK;
 }
@@ -583,8 +665,9 @@ meso_status meso_scene_update_bricks(meso_scene *scene, const uint32_t *d_slots,
   MESO_HIP(hipMallocAsync((void **)&bad, 16, s));
   meso_status st = launch_zero(bad, 16, s);
   if (st != MESO_OK) return st;
-  hipLaunchKernelGGL(brick_update_kernel, dim3((uint32_t)n_edit), dim3(512), 0, s, d_slots,
-                     d_voxels, scene->n_bricks, scene->masks, scene->voxels, bad);
+  hipLaunchKernelGGL(brick_update_kernel, grid_1d((uint64_t)n_edit), dim3(512), 0, s, d_slots,
+                     d_voxels, scene->n_bricks, scene->masks, scene->voxels, bad,
+                     (uint64_t)n_edit);
   MESO_HIP(hipGetLastError());
   uint32_t nbad = 0;
   MESO_HIP(hipMemcpyAsync(&nbad, bad, 4, hipMemcpyDeviceToHost, s));

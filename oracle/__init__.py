@@ -107,6 +107,8 @@ def lib():
             "mo_ref_test_generator": (C.c_int32, [C.c_int32] * 3 + [C.c_float, C.c_int32, P]),
             "mo_ref_octant_id": (C.c_int32, [C.c_float] * 3),
             "mo_build_coarse": (None, [P, C.c_int64, P, P, P]),
+            "mo_edit_chunks": (None, [P, P, P, P, C.c_int32, C.c_uint32]),
+            "mo_scene_chunk": (None, [P, C.c_int32, C.c_int32, C.c_int32, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -397,3 +399,15 @@ def dense_from_bricks(n, slots, voxels):
         g[bz * 8:bz * 8 + 8, by * 8:by * 8 + 8, bx * 8:bx * 8 + 8] = \
             voxels[s[bz, by, bx]].reshape(8, 8, 8)
     return g
+
+
+def edit_chunks(scene, chunk_xyz, seed):
+    """Applies the seeded chunk edits to the scene's voxel and mask arrays in place."""
+    xyz = np.ascontiguousarray(chunk_xyz, np.int32).ravel()
+    lib().mo_edit_chunks(scene.ptr, scene.s.voxels, scene.s.masks, _p(xyz), len(xyz) // 3, seed)
+
+
+def scene_chunk(scene, cx, cy, cz):
+    out = np.zeros(32768, np.uint16)
+    lib().mo_scene_chunk(scene.ptr, cx, cy, cz, _p(out))
+    return out

@@ -1120,3 +1120,63 @@ int32_t mo_ref_octant_id(float x, float y, float z) {
   if (z < 0.0f) id |= 4;
   return id;
 }
+
+/* ------------------------------------------------------------------ */
+/* edits and scene chunks                                               */
+/* ------------------------------------------------------------------ */
+
+/* slot of the brick (bx, by, bz) of a brick map or two-level scene */
+static uint32_t brick_slot(const mo_scene *s, int32_t bx, int32_t by, int32_t bz) {
+  const int32_t nb = s->n / 8;
+  if (bx < 0 || by < 0 || bz < 0 || bx >= nb || by >= nb || bz >= nb) return MO_EMPTY_SLOT;
+  if (s->kind == MO_SCENE_BRICKMAP_U16) return s->slots[lin3(bx, by, bz, nb)];
+  const uint32_t node = s->top[lin3(bx >> 4, by >> 4, bz >> 4, s->n / 128)];
+  if (node == MO_EMPTY_SLOT) return MO_EMPTY_SLOT;
+  return s->slots[(int64_t)node * 4096 + lin3(bx & 15, by & 15, bz & 15, 16)];
+}
+
+void mo_edit_chunks(const mo_scene *s, uint16_t *voxels, uint64_t *masks, const int32_t *xyz,
+                    int32_t n_chunks, uint32_t seed) {
+  const int32_t nc = (s->n + 31) / 32; /* a partial last chunk counts */
+  for (int32_t k = 0; k < n_chunks; k++) {
+    const int32_t cx = xyz[3 * k], cy = xyz[3 * k + 1], cz = xyz[3 * k + 2];
+    if (cx < 0 || cy < 0 || cz < 0 || cx >= nc || cy >= nc || cz >= nc) continue;
+    const uint32_t h0 = mo_hash32(seed ^ mo_hash32((uint32_t)lin3(cx, cy, cz, nc)));
+    const uint32_t h1 = mo_hash32(h0), h2 = mo_hash32(h1), h3 = mo_hash32(h2);
+    const int64_t sx = (int64_t)cx * 8192 + (h0 & 0x1FFF), sy = (int64_t)cy * 8192 + (h1 & 0x1FFF),
+                  sz = (int64_t)cz * 8192 + (h2 & 0x1FFF);
+    const int64_t r = 4 * 256 + (h3 % (8u * 256u));
+    for (int32_t b = 0; b < 64; b++) {
+      const int32_t bx = cx * 4 + (b & 3), by = cy * 4 + ((b >> 2) & 3), bz = cz * 4 + (b >> 4);
+      const uint32_t slot = brick_slot(s, bx, by, bz);
+      if (slot == MO_EMPTY_SLOT) continue;
+      for (int32_t i = 0; i < 512; i++) {
+        const int32_t lx = i & 7, ly = (i >> 3) & 7, lz = i >> 6;
+        const int64_t dx = (int64_t)(bx * 8 + lx) * 256 + 128 - sx;
+        const int64_t dy = (int64_t)(by * 8 + ly) * 256 + 128 - sy;
+        const int64_t dz = (int64_t)(bz * 8 + lz) * 256 + 128 - sz;
+        if (dx * dx + dy * dy + dz * dz > r * r) continue;
+        uint16_t *v = voxels + (int64_t)slot * 512 + i;
+        *v = *v ? 0 : 7;
+      }
+      for (int32_t lz = 0; lz < 8; lz++) {
+        uint64_t m = 0;
+        for (int32_t j = 0; j < 64; j++)
+          if (voxels[(int64_t)slot * 512 + lz * 64 + j]) m |= 1ull << j;
+        masks[(int64_t)slot * 8 + lz] = m;
+      }
+    }
+  }
+}
+
+void mo_scene_chunk(const mo_scene *s, int32_t cx, int32_t cy, int32_t cz, uint16_t *out) {
+  for (int32_t z = 0; z < 32; z++)
+    for (int32_t y = 0; y < 32; y++)
+      for (int32_t x = 0; x < 32; x++) {
+        const int32_t gx = cx * 32 + x, gy = cy * 32 + y, gz = cz * 32 + z;
+        const uint32_t slot = brick_slot(s, gx >> 3, gy >> 3, gz >> 3);
+        out[x + 32 * y + 1024 * z] =
+            slot == MO_EMPTY_SLOT ? 0
+                                  : s->voxels[(int64_t)slot * 512 + (gx & 7) + 8 * (gy & 7) + 64 * (gz & 7)];
+      }
+}

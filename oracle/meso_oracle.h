@@ -228,6 +228,16 @@ int32_t mo_ref_test_generator(int32_t cx, int32_t cy, int32_t cz, float block_si
 /* GetOctantId (Samples/SimpleVoxel.cpp:129-136) */
 int32_t mo_ref_octant_id(float x, float y, float z);
 
+/* ---------------- edits and scene chunks ---------------- */
+/* Seeded edit of 32^3 chunks (chunk coordinates xyz[3k]): inside a sphere
+ * drawn per chunk from lowbias32(seed ^ lowbias32(chunk linear index)) every
+ * voxel of an allocated brick toggles (solid -> 0, empty -> 7); masks follow.
+ * Bricks are never allocated or freed.  voxels / masks are modified in place. */
+void mo_edit_chunks(const mo_scene *s, uint16_t *voxels, uint64_t *masks, const int32_t *xyz,
+                    int32_t n_chunks, uint32_t seed);
+/* The 32^3 u16 voxels of chunk (cx, cy, cz) of a brick scene (x fastest). */
+void mo_scene_chunk(const mo_scene *s, int32_t cx, int32_t cy, int32_t cz, uint16_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,227 @@
+"""GPU tests of the rest of the C-ABI: dense-grid chunk meshing, graph replay,
+the autotuner, the reference sphere voxelizer with its erosion chain, brick
+edits and argument errors."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from mesoengine_amd import _lib as L
+from mesoengine_amd import runtime as R
+from mesoengine_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def tiles32(grid):
+    """[z][y][x] dense grid -> chunks in linear chunk order (cx fastest)"""
+    n = grid.shape[0]
+    nc = n // 32
+    out = []
+    for cz in range(nc):
+        for cy in range(nc):
+            for cx in range(nc):
+                out.append(np.ascontiguousarray(
+                    grid[cz * 32:(cz + 1) * 32, cy * 32:(cy + 1) * 32, cx * 32:(cx + 1) * 32]).ravel())
+    return out
+
+
+@pytest.mark.parametrize("mode", [L.MESH_CULLED, L.MESH_GREEDY])
+def test_dense_grid_u8_chunks(ctx, mode):
+    # the config-1 128^3 u8 sphere grid meshed as 4^3 chunks of 32^3
+    g = O.spheres_dense(W.SPHERES["seed"], 128, 8)
+    buf = ctx.upload(g.ravel())
+    chunks = tiles32(g)
+    refs = [O.mesh_chunk(c, mode == L.MESH_GREEDY) for c in chunks]
+    total = sum(len(r) for r in refs)
+    mt = R.MeshTargets(ctx, 64, total)
+    R.mesh_chunks(ctx, R.chunk_source(buf, 1, 64, L.CHUNKS_DENSE_GRID, 128), mode, mt)
+    ctx.sync()
+    counts, prefix, tot, quads = mt.results()
+    assert tot == total
+    for c in range(64):
+        assert np.array_equal(quads[prefix[c]:prefix[c] + counts[c]], refs[c]), c
+
+
+def test_graph_replay_matches_eager(ctx):
+    n = 128
+    vox = ctx.buffer(2 * 32768 * n)
+    L.check(ctx.lib.meso_generate_chunks(ctx.h, 31, n, vox.ptr, None))
+    src = R.chunk_source(vox, 2, n)
+    mt = R.MeshTargets(ctx, n, 4 << 20)
+    R.mesh_chunks(ctx, src, L.MESH_GREEDY, mt)
+    ctx.sync()
+    eager = mt.results()
+    g = C.c_void_p()
+    gt = R.MeshTargets(ctx, n, 4 << 20)
+    L.check(ctx.lib.meso_mesh_graph_create(ctx.h, L.byref(src), L.MESH_GREEDY, gt.counts.ptr,
+                                           gt.prefix.ptr, gt.quads.ptr, gt.capacity, gt.total.ptr,
+                                           gt.work.ptr, C.byref(g)))
+    for rep in range(3):
+        gt.counts.zero()
+        gt.quads.zero()
+        L.check(ctx.lib.meso_graph_launch(g, None))
+        ctx.sync()
+        got = gt.results()
+        for a, b in zip(eager, got):
+            assert np.array_equal(np.asarray(a), np.asarray(b)), rep
+        # eager work between replays
+        R.mesh_chunks(ctx, src, L.MESH_CULLED, mt)
+        ctx.sync()
+    L.check(ctx.lib.meso_graph_destroy(g))
+    # plain graph: one launch node per chunk
+    per = int(eager[0].max())
+    pc, pq = ctx.buffer(4 * n), ctx.buffer(4 * per * n)
+    L.check(ctx.lib.meso_mesh_plain_graph_create(ctx.h, L.byref(src), L.MESH_GREEDY, pc.ptr, pq.ptr,
+                                                 per, C.byref(g)))
+    for rep in range(2):
+        L.check(ctx.lib.meso_graph_launch(g, None))
+        ctx.sync()
+        assert np.array_equal(pc.download(np.uint32), eager[0])
+    q = pq.download(np.uint32).reshape(n, per)
+    for c in range(0, n, 9):
+        k = int(eager[0][c])
+        assert np.array_equal(q[c, :k], eager[3][eager[1][c]:eager[1][c] + k])
+    L.check(ctx.lib.meso_graph_destroy(g))
+
+
+def test_autotuners_keep_results(ctx, tmp_path):
+    n = 64
+    vox = ctx.buffer(2 * 32768 * n)
+    L.check(ctx.lib.meso_generate_chunks(ctx.h, 32, n, vox.ptr, None))
+    src = R.chunk_source(vox, 2, n)
+    mt = R.MeshTargets(ctx, n, 2 << 20)
+    R.mesh_chunks(ctx, src, L.MESH_GREEDY, mt)
+    ctx.sync()
+    before = mt.results()
+    tr = L.TuneResult()
+    tr.size = C.sizeof(tr)
+    L.check(ctx.lib.meso_autotune_mesh(ctx.h, L.byref(src), L.MESH_GREEDY, mt.counts.ptr,
+                                       mt.prefix.ptr, mt.quads.ptr, mt.capacity, mt.total.ptr,
+                                       mt.work.ptr, L.byref(tr)))
+    assert tr.tried >= 5 and tr.best_ms > 0 and tr.best_ms <= tr.default_ms * 1.01
+    R.mesh_chunks(ctx, src, L.MESH_GREEDY, mt)  # now with the tuned shape
+    ctx.sync()
+    for a, b in zip(before, mt.results()):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    sc = ctx.scene_generate(L.GEN_SPHERES, L.SCENE_DENSE_U8, 128, 1, 8)
+    cp = W.spheres_camera(width=200, height=120)
+    cam = R.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], 200, 120)
+    tg = R.RayTargets(ctx, 200, 120)
+    R.raymarch(ctx, sc, cam, L.RAY_SHADOW, R.shading(), tg)
+    ctx.sync()
+    ref = tg.results()
+    rr = L.TuneResult()
+    rr.size = C.sizeof(rr)
+    L.check(ctx.lib.meso_autotune_raymarch(ctx.h, sc.h, L.byref(cam), L.RAY_SHADOW,
+                                           L.byref(R.shading()), L.byref(tg.out), L.byref(rr)))
+    assert rr.tried == 5 and rr.tile_w * rr.tile_h == rr.block
+    R.raymarch(ctx, sc, cam, L.RAY_SHADOW, R.shading(), tg)
+    ctx.sync()
+    for a, b in zip(ref, tg.results()):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    path = str(tmp_path / "tune.txt").encode()
+    L.check(ctx.lib.meso_autotune_save(ctx.h, path))
+    assert os.path.getsize(path) > 0
+    L.check(ctx.lib.meso_autotune_load(ctx.h, path))
+    sc.destroy()
+
+
+def test_reference_sphere_chunks_and_erosion(ctx):
+    coords = np.array([(cx, cy, cz) for cz in range(-4, 4) for cy in range(-4, 4)
+                       for cx in range(3, 10)], np.int32)
+    nc = len(coords)
+    dc = ctx.upload(coords.ravel())
+    mips = ctx.buffer(nc * 4 * 4096)
+    L.check(ctx.lib.meso_generate_sphere_chunks(ctx.h, dc.ptr, nc, 16, 1.0, 4, mips.ptr, None))
+    ctx.sync()
+    got = mips.download(np.uint8).reshape(nc, 4, 16, 16, 16)
+    solid = 0
+    for i, (cx, cy, cz) in enumerate(coords):
+        cells = O.ref_generate_sphere(int(cx), int(cy), int(cz), 1.0, 16)
+        occ = np.zeros((16, 16, 16), np.uint8)
+        if len(cells):
+            occ[cells[:, 2], cells[:, 1], cells[:, 0]] = 1
+        solid += len(cells)
+        assert np.array_equal(got[i], O.ref_erode_mips(occ, 4)), (cx, cy, cz)
+    xs = np.arange(48, 160)[:, None, None]
+    ys = np.arange(-64, 64)[None, :, None]
+    zs = np.arange(-64, 64)[None, None, :]
+    assert solid == int(((xs - 100) ** 2 + ys ** 2 + zs ** 2 < 2500).sum())
+
+
+def test_update_bricks(ctx):
+    n = 128
+    sc = ctx.scene_generate(L.GEN_TERRAIN_CAVES, L.SCENE_BRICKMAP_U16, n, 60)
+    a = sc.arrays()
+    nb = len(a["masks"])
+    rng = np.random.default_rng(5)
+    sel = rng.choice(nb, 20, replace=False).astype(np.uint32)
+    edits = rng.integers(0, 4, (20, 512)).astype(np.uint16) * (rng.random((20, 512)) < 0.5)
+    edits[0] = 0  # a brick emptied by the edit keeps its slot with a zero mask
+    ds, dv = ctx.upload(sel), ctx.upload(edits.astype(np.uint16).ravel())
+    L.check(ctx.lib.meso_scene_update_bricks(sc.h, ds.ptr, dv.ptr, 20, None))
+    b = sc.arrays()
+    vox = a["voxels"].copy()
+    vox[sel] = edits
+    masks = a["masks"].copy()
+    for i, s in enumerate(sel):
+        v = edits[i].reshape(8, 8, 8)
+        for lz in range(8):
+            m = 0
+            for ly in range(8):
+                for lx in range(8):
+                    if v[lz, ly, lx]:
+                        m |= 1 << (lx + 8 * ly)
+            masks[s, lz] = m
+    assert np.array_equal(b["voxels"], vox) and np.array_equal(b["masks"], masks)
+    assert np.array_equal(b["slots"], a["slots"])
+    cp = dict(eye=[-30.0, 120.0, -20.0], target=[64.0, 20.0, 64.0], up=[0.0, 1.0, 0.0], vfov=60.0)
+    cam = R.camera(cp["eye"], cp["target"], cp["up"], 60.0, 160, 100)
+    tg = R.RayTargets(ctx, 160, 100)
+    R.raymarch(ctx, sc, cam, L.RAY_SHADOW, R.shading(W.LIGHT), tg)
+    ctx.sync()
+    ocam = O.camera(cp["eye"], cp["target"], cp["up"], 60.0, 160, 100)
+    ref = O.raymarch(O.brickmap_scene(n, b["slots"], b["masks"], b["voxels"]), ocam, O.RAY_SHADOW,
+                     O.shading(W.LIGHT))
+    for x, y in zip(tg.results(), ref[:4]):
+        assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
+    bad = ctx.upload(np.array([nb + 5], np.uint32))
+    with pytest.raises(L.MesoError) as e:
+        L.check(ctx.lib.meso_scene_update_bricks(sc.h, bad.ptr, dv.ptr, 1, None))
+    assert e.value.status == L.ERR_ARG
+    sc.destroy()
+
+
+def test_argument_errors(ctx):
+    lib = ctx.lib
+    sc = ctx.scene_generate(L.GEN_SPHERES, L.SCENE_DENSE_U8, 64, 3, 4)
+    tg = R.RayTargets(ctx, 8, 8)
+    cam = R.camera((100, 100, 100), (32, 32, 32), (0, 0, 1), 60.0, 8, 8)
+    bad_up = R.camera((0, 0, 100), (0, 0, 0), (0, 0, 1), 60.0, 8, 8)
+    assert lib.meso_raymarch(ctx.h, sc.h, L.byref(bad_up), 0, None, L.byref(tg.out), None) == L.ERR_ARG
+    tg.out.index_bytes = 3
+    assert lib.meso_raymarch(ctx.h, sc.h, L.byref(cam), 0, None, L.byref(tg.out), None) == L.ERR_ARG
+    tg.out.index_bytes = 4
+    sh = R.shading(ao_samples=0)
+    assert lib.meso_raymarch(ctx.h, sc.h, L.byref(cam), L.RAY_AO, L.byref(sh), L.byref(tg.out),
+                             None) == L.ERR_ARG
+    assert lib.meso_raymarch(ctx.h, sc.h, L.byref(cam), L.RAY_COUNT, None, L.byref(tg.out),
+                             None) == L.ERR_ARG
+    buf = ctx.buffer(2 * 32768 + 64)
+    cnt = ctx.buffer(4)
+    assert lib.meso_mesh_chunk(ctx.h, buf.ptr + 2, 2, L.MESH_GREEDY, cnt.ptr, None, 0,
+                               None) == L.ERR_ARG
+    slots = np.full(8 ** 3, 0xFFFFFFFF, np.uint32)
+    slots[5] = 3  # names a brick that does not exist
+    h = C.c_void_p()
+    m = np.zeros((1, 8), np.uint64)
+    v = np.zeros((1, 512), np.uint16)
+    assert lib.meso_scene_create_brickmap_u16(ctx.h, 64, slots.ctypes.data, 1, m.ctypes.data,
+                                              v.ctypes.data, C.byref(h)) == L.ERR_ARG
+    assert b"slot" in lib.meso_last_error_message()
+    assert lib.meso_buffer_upload(buf.h, m.ctypes.data, 1 << 30, 0) == L.ERR_ARG
+    assert lib.meso_buffer_device_address(buf.h, 3) == 0
+    sc.destroy()

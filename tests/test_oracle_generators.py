@@ -3,6 +3,7 @@ import ctypes as C
 import math
 
 import numpy as np
+import pytest
 
 import oracle as O
 
@@ -133,6 +134,51 @@ def test_chunk_noise_threshold_meets_target():
         solid = int((vox > 0).sum())
         assert solid >= target
         assert set(np.unique(vox[vox > 0]).tolist()) <= {1, 2, 3, 4}
+
+
+@pytest.mark.parametrize("n", [128, 200])
+def test_scene_chunk_and_edits_match_dense_restatement(n):
+    nc = -(-n // 32)  # a partial last chunk counts
+    slots, masks, vox = O.build_brickmap(O.GEN_TERRAIN_CAVES, n, 41)
+    masks, vox = masks.copy(), vox.copy()
+    sc = O.brickmap_scene(n, slots, masks, vox)
+    dense = O.dense_from_bricks(n, slots, vox)
+    pad = np.zeros((nc * 32,) * 3, dense.dtype)
+    pad[:n, :n, :n] = dense
+    for c in [(0, 0, 0), (3, 1, 2), (1, 3, 3), (nc - 1, nc - 1, 0)]:
+        assert np.array_equal(O.scene_chunk(sc, *c).reshape(32, 32, 32),
+                              pad[c[2] * 32:(c[2] + 1) * 32, c[1] * 32:(c[1] + 1) * 32,
+                                  c[0] * 32:(c[0] + 1) * 32])
+    xyz = np.array([[0, 0, 0], [2, 1, 3], [3, 0, 1], [nc - 1, 1, nc - 1], [nc, 0, 0]], np.int32)
+    O.edit_chunks(sc, xyz, 77)
+    # restatement: toggle every voxel of an allocated brick inside the chunk's sphere
+    H = O.lib().mo_hash32
+    alloc = np.zeros((n, n, n), bool)
+    s3 = slots.reshape(n // 8, n // 8, n // 8)
+    for bz, by, bx in zip(*np.nonzero(s3 != O.EMPTY_SLOT)):
+        alloc[bz * 8:bz * 8 + 8, by * 8:by * 8 + 8, bx * 8:bx * 8 + 8] = True
+    z, y, x = np.indices((n, n, n), dtype=np.int64)
+    exp = dense.copy()
+    for cx, cy, cz in xyz:
+        if max(cx, cy, cz) >= nc:
+            continue  # outside the chunk grid: skipped
+        h0 = H(77 ^ H(int(cx + nc * (cy + nc * cz))))
+        h1 = H(h0)
+        h2 = H(h1)
+        h3 = H(h2)
+        sx, sy, sz = cx * 8192 + (h0 & 0x1FFF), cy * 8192 + (h1 & 0x1FFF), cz * 8192 + (h2 & 0x1FFF)
+        r = 4 * 256 + h3 % (8 * 256)
+        inside = ((x * 256 + 128 - sx) ** 2 + (y * 256 + 128 - sy) ** 2 + (z * 256 + 128 - sz) ** 2
+                  <= r * r) & alloc & (x // 32 == cx) & (y // 32 == cy) & (z // 32 == cz)
+        exp[inside] = np.where(exp[inside] != 0, 0, 7)
+    assert np.array_equal(O.dense_from_bricks(n, slots, vox), exp)
+    s2, m2, v2 = O.brickmap_from_dense(exp)
+    # masks follow the voxels; bricks that lost every voxel keep their slot
+    ref_m = np.zeros_like(masks)
+    for b in range(len(vox)):
+        v = vox[b].reshape(8, 64)
+        ref_m[b] = [sum(1 << j for j in range(64) if v[lz, j]) for lz in range(8)]
+    assert np.array_equal(masks, ref_m)
 
 
 def test_two_level_node_major_order():
