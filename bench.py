@@ -5,8 +5,10 @@ Headline (value): Mrays/s of one 1920x1080 frame of primary rays plus one
 shadow ray per lit hit over the 512^3 heightfield brick map (u16 materials,
 8^3 bricks), scene and outputs resident in HBM.  Beside it in the same JSON
 line: config-3 meshing (4096 chunks of 32^3 u16) batched vs plain, serving
-(continuous batching over 16 resident scenes at a fixed QPS, p50/p99) and the
-scalar CPU oracle timed on this host's cores.
+(continuous batching over 16 resident scenes at a fixed QPS, p50/p99), the
+2048^3 cave frame loop (1% of chunks edited and remeshed, then a 3840x2160
+frame with shadow and AO rays; alone and under serving load) and the scalar
+CPU oracle timed on this host's cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -206,6 +208,9 @@ def main():
     ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-frame", action="store_true",
+                    help="skip the 2048^3 cave frame loop (edit + remesh + render)")
+    ap.add_argument("--frames", type=int, default=8)
     ap.add_argument("--no-graph", action="store_true",
                     help="batched meshing launched directly instead of by graph replay")
     ap.add_argument("--serve-requests", type=int, default=400)
@@ -232,12 +237,20 @@ def main():
     serve = None
     if not args.no_serve:
         serve = serve_leg(ctx, R, L, W, args.serve_requests, args.qps, args.budget_mb)
+    frame = None
+    if not args.no_frame:
+        from mesoengine_amd import frame as F
+        qps = serve["qps"] if serve else (args.qps or 1000.0)
+        frame = F.run(ctx, frames=args.frames, warmup=2, serve_qps=qps,
+                      serve_requests=args.serve_requests, budget_bytes=args.budget_mb << 20,
+                      log=log)
     cpu = None if args.no_cpu else cpu_leg(W)
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tp):
         with open(tp) as f:
-            traffic = json.load(f).get("raymarch_brickmap_terrain512_hbm_bytes_per_launch")
+            k = json.load(f).get("kernels", {}).get("raymarch_brickmap_terrain512")
+            traffic = k["hbm_bytes"] if k else None
     line = {
         "metric": "Mrays/s and p99 latency at fixed QPS; Mvoxels meshed/s; batched vs plain",
         "value": round(rm["mrays"], 2),
@@ -262,6 +275,7 @@ def main():
         "mrays_primary": round(rm["mrays_primary"], 2),
         "meshing": mesh,
         "serving": serve,
+        "frame": frame,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -116,6 +116,12 @@ meso_status meso_memcpy_d2h(meso_context *ctx, void *dst, const void *src, uint6
                             meso_stream stream);
 meso_status meso_memcpy_h2d(meso_context *ctx, void *dst, const void *src, uint64_t bytes,
                             meso_stream stream);
+/* device-to-device copy, asynchronous on the stream.  lvk has no buffer to
+ * buffer copy (only IContext::upload, LVK.h:822); this one moves results
+ * between device buffers and is the known-size copy that calibrates the
+ * HBM traffic counters. */
+meso_status meso_memcpy_d2d(meso_context *ctx, void *dst, const void *src, uint64_t bytes,
+                            meso_stream stream);
 
 /* ------------------------------------------------------------------ */
 /* scenes: the engine's voxel data model (FVoxelSceneConfig,             */

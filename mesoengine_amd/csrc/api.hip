@@ -414,6 +414,14 @@ meso_status meso_memcpy_h2d(meso_context *ctx, void *dst, const void *src, uint6
   return MESO_OK;
 }
 
+meso_status meso_memcpy_d2d(meso_context *ctx, void *dst, const void *src, uint64_t bytes,
+                            meso_stream stream) {
+  MESO_CHECK(ctx && dst && src, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  hipStream_t s = stream_of(ctx, stream);
+  MESO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+  return MESO_OK;
+}
+
 // ---------------- scenes
 static meso_status scene_alloc(meso_context *ctx, void **p, uint64_t bytes, const void *src,
                                meso_scene *sc) {

@@ -449,14 +449,19 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_batched_kernel(
     uint32_t excl = 0;
     const uint32_t chunk_total = block_exclusive_scan(L, mine, excl);
     if (t == 0) {
-      // publish the aggregate, then add up predecessors until an inclusive one
-      __hip_atomic_store(&work->status[c], kAgg | (uint64_t)chunk_total, __ATOMIC_RELEASE,
+      // publish the aggregate, then add up predecessors until an inclusive one.
+      // The value travels inside the status word and no other data is
+      // exchanged, so the status accesses are relaxed: agent-scope release /
+      // acquire would write back and invalidate this XCD's whole L2 at every
+      // chunk (and every spin), evicting the chunk data of the other
+      // workgroups on it.
+      __hip_atomic_store(&work->status[c], kAgg | (uint64_t)chunk_total, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
       uint64_t before = 0;
       int64_t j = (int64_t)c - 1;
       uint32_t spins = 0;
       while (j >= 0) {
-        const uint64_t st = __hip_atomic_load(&work->status[j], __ATOMIC_ACQUIRE,
+        const uint64_t st = __hip_atomic_load(&work->status[j], __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
         if (st & kInc) {
           before += st & kValMask;
@@ -475,7 +480,7 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_batched_kernel(
         }
         __builtin_amdgcn_s_sleep(1);
       }
-      __hip_atomic_store(&work->status[c], kInc | (before + chunk_total), __ATOMIC_RELEASE,
+      __hip_atomic_store(&work->status[c], kInc | (before + chunk_total), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
       counts[c] = chunk_total;
       prefix[c] = (uint32_t)before;

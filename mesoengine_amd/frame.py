@@ -1,0 +1,177 @@
+"""Frame loop over the 2048^3 two-level cave volume.
+
+Each frame:
+  1. edit: 1% of the 32^3 chunks (2,621 of 262,144, a seeded set per frame)
+     receive seeded sphere-toggle edits in place (meso_scene_edit_chunks);
+  2. remesh: the dirty chunks are greedy-meshed straight from the resident
+     scene in one persistent launch (meso_mesh_scene_chunks);
+  3. render: one 3840x2160 frame of primary rays + a shadow ray per lit hit +
+     4 cosine-hemisphere AO rays per hit (meso_raymarch with SHADOW | AO).
+The three steps run back to back on the context stream and are timed with
+events between them.  run() times the frame alone, then again while a
+serving session (the 16-scene request stream of the serving workload) runs
+on the server's own streams, and reports the frame split and the served
+latency under that load.
+"""
+import threading
+import time
+
+import numpy as np
+
+from . import _lib as L
+from . import runtime as R
+from . import serving
+from . import workloads as W
+
+
+def dirty_chunks(n, frame, seed, frac=0.01):
+    """Seeded set of unique dirty chunks for one frame, int32 (x, y, z) rows."""
+    nc = -(-n // 32)
+    total = nc ** 3
+    count = int(total * frac)
+    rng = np.random.default_rng([seed, frame])
+    flat = rng.choice(total, size=count, replace=False)
+    return np.stack([flat % nc, (flat // nc) % nc, flat // (nc * nc)], 1).astype(np.int32)
+
+
+def frame_bytes(cnt, index_bytes=8):
+    """Algorithmic bytes of one rendered frame: 4 B per top-level and node
+    pointer read, 64 B per brick mask fetched, 2 B material per hit, and per
+    pixel the outputs (index, 1 B face, 4 B t, 4 B shade)."""
+    return (4 * cnt["n_top"] + 4 * cnt["n_node"] + 64 * cnt["n_brickmask"] + 2 * cnt["hits"]
+            + (index_bytes + 9) * cnt["rays"])
+
+
+class _Frames:
+    def __init__(self, ctx, cfg, seed, log):
+        self.ctx = ctx
+        self.cfg = cfg
+        n = cfg["n"]
+        t = time.time()
+        self.scene = ctx.scene_generate(L.GEN_CAVES, L.SCENE_TWO_LEVEL_U16, n, cfg["seed"])
+        ctx.sync()
+        i = self.scene.info()
+        self.gen_s = time.time() - t
+        self.info = dict(n=n, bricks=int(i.n_bricks), nodes=int(i.n_nodes),
+                         device_mb=round(i.device_bytes / 1e6, 1),
+                         fill=round(i.solid_voxels / n ** 3, 5), generate_s=round(self.gen_s, 3))
+        log("frame scene %d^3 caves: %d bricks in %d nodes, %.0f MB, %.4f filled, built in %.2fs" % (
+            n, i.n_bricks, i.n_nodes, i.device_bytes / 1e6, i.solid_voxels / n ** 3, self.gen_s))
+        cp = W.caves_camera(n, cfg["width"], cfg["height"])
+        self.cam = R.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], cfg["width"],
+                            cfg["height"])
+        self.sh = R.shading(W.LIGHT, seed, cfg["ao_samples"], cfg["ao_radius"])
+        self.flags = L.RAY_SHADOW | L.RAY_AO
+        self.tg = R.RayTargets(ctx, cfg["width"], cfg["height"], index_bytes=8, counters=True)
+        self.seed = seed
+        self.n_dirty = len(dirty_chunks(n, 0, seed))
+        self.mt = R.MeshTargets(ctx, self.n_dirty, 24 << 20)
+        self.ev = [R.Event(ctx) for _ in range(4)]
+
+    def count(self):
+        R.raymarch(self.ctx, self.scene, self.cam, self.flags | L.RAY_COUNT, self.sh, self.tg)
+        self.ctx.sync()
+        return self.tg.counts()
+
+    def run(self, frames, first=0):
+        """Time `frames` frames; returns per-frame (edit, remesh, render) ms and quads."""
+        lists = [self.ctx.upload(dirty_chunks(self.cfg["n"], first + f, self.seed).ravel())
+                 for f in range(frames)]
+        self.ctx.sync()
+        out = []
+        e = self.ev
+        for f in range(frames):
+            e[0].record()
+            R.edit_chunks(self.scene, lists[f], self.n_dirty, self.seed * 7919 + first + f)
+            e[1].record()
+            R.mesh_scene_chunks(self.ctx, self.scene, lists[f], self.n_dirty, L.MESH_GREEDY,
+                                self.mt)
+            e[2].record()
+            R.raymarch(self.ctx, self.scene, self.cam, self.flags, self.sh, self.tg)
+            e[3].record()
+            e[3].sync()
+            total = int(self.mt.total.download(np.uint64, 1)[0])
+            out.append((e[0].elapsed_ms(e[1]), e[1].elapsed_ms(e[2]), e[2].elapsed_ms(e[3]),
+                        total))
+        return out
+
+    def close(self):
+        self.scene.destroy()
+
+
+def _summary(rows):
+    a = np.array([r[:3] for r in rows], np.float64)
+    med = np.median(a, axis=0)
+    return dict(frames=len(rows), edit_ms=round(float(med[0]), 4),
+                remesh_ms=round(float(med[1]), 4), render_ms=round(float(med[2]), 4),
+                frame_ms=round(float(np.median(a.sum(1))), 4),
+                frame_ms_max=round(float(a.sum(1).max()), 4),
+                quads_per_frame=int(np.median([r[3] for r in rows])))
+
+
+def run(ctx, frames=8, warmup=2, serve_qps=1000.0, serve_requests=400, budget_bytes=512 << 20,
+        seed=5, log=print):
+    cfg = W.CAVES
+    fr = _Frames(ctx, cfg, seed, log)
+    try:
+        cnt = fr.count()
+        rays = cnt["rays"] + cnt["shadow_rays"] + cnt["ao_rays"]
+        alg = frame_bytes(cnt)
+        fr.run(warmup, first=10_000)
+        alone = _summary(fr.run(frames))
+        alone["render_mrays"] = round(rays / (alone["render_ms"] * 1e-3) / 1e6, 1)
+        alone["render_alg_gbs"] = round(alg / (alone["render_ms"] * 1e-3) / 1e9, 1)
+        remesh_bytes = 65536 * fr.n_dirty + 4 * alone["quads_per_frame"] + 8 * fr.n_dirty
+        alone["remesh_mvox_s"] = round(fr.n_dirty * 32768 / (alone["remesh_ms"] * 1e-3) / 1e6, 1)
+        alone["remesh_alg_gbs"] = round(remesh_bytes / (alone["remesh_ms"] * 1e-3) / 1e9, 1)
+        log("frame alone: edit %.3f ms, remesh %.3f ms, render %.3f ms (%.0f Mrays/s), "
+            "frame %.3f ms" % (alone["edit_ms"], alone["remesh_ms"], alone["render_ms"],
+                               alone["render_mrays"], alone["frame_ms"]))
+
+        # the same frames while the serving stream runs on the server's streams
+        edges = W.serve_scenes(4)
+        scenes = serving.make_scenes(ctx, edges, 4)
+        reqs = W.serve_requests(edges, serve_requests, serve_qps, 4)
+        sh = R.shading(W.LIGHT)
+        result = {}
+
+        def serve():
+            try:
+                result["lat"], result["makespan"], result["stats"] = serving.served_run(
+                    ctx, scenes, reqs, sh, budget_bytes, True)
+            except Exception as ex:  # surfaced after join
+                result["error"] = ex
+
+        th = threading.Thread(target=serve)
+        th.start()
+        rows = []
+        f = 0
+        while th.is_alive() or len(rows) < frames:
+            rows += fr.run(1, first=20_000 + f)
+            f += 1
+            if len(rows) >= 4 * frames and not th.is_alive():
+                break
+        th.join()
+        if "error" in result:
+            raise result["error"]
+        for s in scenes:
+            s.destroy()
+        loaded = _summary(rows)
+        loaded["render_mrays"] = round(rays / (loaded["render_ms"] * 1e-3) / 1e6, 1)
+        lat = result["lat"]
+        loaded["serving"] = dict(qps=serve_qps, requests=serve_requests,
+                                 p50_ms=round(float(np.percentile(lat, 50) * 1e3), 3),
+                                 p99_ms=round(float(np.percentile(lat, 99) * 1e3), 3),
+                                 launches=int(result["stats"]["launches"]))
+        log("frame under serving load (%.0f QPS): edit %.3f ms, remesh %.3f ms, render %.3f ms, "
+            "frame %.3f ms (max %.3f); served p50 %.2f p99 %.2f ms" % (
+                serve_qps, loaded["edit_ms"], loaded["remesh_ms"], loaded["render_ms"],
+                loaded["frame_ms"], loaded["frame_ms_max"], loaded["serving"]["p50_ms"],
+                loaded["serving"]["p99_ms"]))
+        return dict(workload="2048^3 two-level caves (u16, 8^3 bricks under 16^3-brick nodes), "
+                             "3840x2160 primary + shadow + 4 AO rays per hit; 1% of 32^3 chunks "
+                             "edited and greedy-remeshed per frame",
+                    scene=fr.info, dirty_chunks=fr.n_dirty, counts=cnt, rays_per_frame=rays,
+                    render_alg_bytes=alg, alone=alone, under_serving_load=loaded)
+    finally:
+        fr.close()

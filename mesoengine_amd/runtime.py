@@ -336,3 +336,16 @@ class Server:
         if self.h:
             check(self.ctx.lib.meso_serve_stop(self.h))
             self.h = None
+
+
+def edit_chunks(scene, xyz_buf, n_chunks, seed, stream=None):
+    """Seeded sphere-toggle edits of the 32^3 chunks listed (int32 xyz triples
+    on the device)."""
+    check(scene.ctx.lib.meso_scene_edit_chunks(scene.h, xyz_buf.ptr, n_chunks, seed, stream))
+
+
+def mesh_scene_chunks(ctx, scene, xyz_buf, n_chunks, mode, mt, stream=None):
+    """Batched meshing of chunks read straight from a resident scene."""
+    check(ctx.lib.meso_mesh_scene_chunks(ctx.h, scene.h, xyz_buf.ptr, n_chunks, mode,
+                                         mt.counts.ptr, mt.prefix.ptr, mt.quads.ptr, mt.capacity,
+                                         mt.total.ptr, mt.work.ptr, stream))

@@ -1,0 +1,154 @@
+"""HBM traffic of the hot kernels from rocprofv3 counters.
+
+Two steps:
+
+  run    the workload profiled under `rocprofv3 --pmc FETCH_SIZE` and again
+         under `--pmc WRITE_SIZE`: three device-to-device copies of a known
+         size (the calibration), three 1920x1080 primary + shadow frames of
+         the 512^3 heightfield brick map, two batched greedy and two batched
+         culled meshings of the 4096-chunk batch, and one 3840x2160 frame
+         (primary + shadow + 4 AO) of the 2048^3 two-level caves;
+  parse  reads both counter CSVs and writes a JSON summary: per kernel the
+         counter's KiB per dispatch, scaled by the factor that maps the
+         copy's reading to the bytes it moved (read = written = copy size).
+
+Usage (rocprofv3 run from /tmp, the program straight after `--`):
+  rocprofv3 --pmc FETCH_SIZE -d OUT/fetch -o run --output-format csv -- \
+      python3 REPO/mesoengine_amd/traffic.py run
+  rocprofv3 --pmc WRITE_SIZE -d OUT/write -o run --output-format csv -- \
+      python3 REPO/mesoengine_amd/traffic.py run
+  python3 REPO/mesoengine_amd/traffic.py parse OUT/fetch OUT/write REPO/profiles/traffic.json
+"""
+import csv
+import glob
+import json
+import os
+import sys
+
+COPY_BYTES = 1 << 30
+
+# kernel-name prefix -> key in the summary
+KERNELS = {
+    "__amd_rocclr_copyBuffer": "copy_calibration",
+    "void meso::raymarch_kernel<2,": "raymarch_brickmap_terrain512",
+    "void meso::raymarch_kernel<3,": "raymarch_two_level_caves2048",
+    "void meso::mesh_batched_kernel<2, true, 0>": "mesh_batched_greedy_4096",
+    "void meso::mesh_batched_kernel<2, false, 0>": "mesh_batched_culled_4096",
+}
+
+
+def run():
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from mesoengine_amd import _lib as L
+    from mesoengine_amd import runtime as R
+    from mesoengine_amd import workloads as W
+
+    ctx = R.Context(0)
+    print("traffic: copy calibration", flush=True)
+    a, b = ctx.buffer(COPY_BYTES), ctx.buffer(COPY_BYTES)
+    a.zero()
+    for _ in range(3):
+        L.check(ctx.lib.meso_memcpy_d2d(ctx.h, b.ptr, a.ptr, COPY_BYTES, None))
+    ctx.sync()
+    a.free()
+    b.free()
+
+    print("traffic: 512^3 heightfield frames", flush=True)
+    cfg = W.TERRAIN
+    sc = ctx.scene_generate(L.GEN_HEIGHTFIELD, L.SCENE_BRICKMAP_U16, cfg["n"], cfg["seed"])
+    cp = W.terrain_camera()
+    cam = R.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], cfg["width"], cfg["height"])
+    tg = R.RayTargets(ctx, cfg["width"], cfg["height"])
+    for _ in range(3):
+        R.raymarch(ctx, sc, cam, L.RAY_SHADOW, R.shading(W.LIGHT), tg)
+    ctx.sync()
+    sc.destroy()
+    del tg
+
+    print("traffic: 4096-chunk batched meshing", flush=True)
+    n_chunks = W.CHUNKS["n_chunks"]
+    vox = ctx.buffer(n_chunks * 32768 * 2)
+    L.check(ctx.lib.meso_generate_chunks(ctx.h, W.CHUNKS["seed"], n_chunks, vox.ptr, None))
+    src = R.chunk_source(vox, 2, n_chunks)
+    mt = R.MeshTargets(ctx, n_chunks, 40 << 20)
+    for mode in (L.MESH_GREEDY, L.MESH_GREEDY, L.MESH_CULLED, L.MESH_CULLED):
+        R.mesh_chunks(ctx, src, mode, mt)
+    ctx.sync()
+    vox.free()
+    del mt
+
+    print("traffic: 2048^3 caves frame", flush=True)
+    cfg = W.CAVES
+    sc = ctx.scene_generate(L.GEN_CAVES, L.SCENE_TWO_LEVEL_U16, cfg["n"], cfg["seed"])
+    cp = W.caves_camera()
+    cam = R.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], cfg["width"], cfg["height"])
+    tg = R.RayTargets(ctx, cfg["width"], cfg["height"], index_bytes=8)
+    R.raymarch(ctx, sc, cam, L.RAY_SHADOW | L.RAY_AO,
+               R.shading(W.LIGHT, 5, cfg["ao_samples"], cfg["ao_radius"]), tg)
+    ctx.sync()
+    sc.destroy()
+    ctx.close()
+    print("traffic: done", flush=True)
+    return 0
+
+
+def _read(folder, counter):
+    """kernel key -> list of per-dispatch counter values (KiB).  Of the copy
+    kernel only the dispatches with the largest grid count: the calibration
+    copies (the library's small uploads use the same kernel)."""
+    files = glob.glob(os.path.join(folder, "**", "*counter_collection*.csv"), recursive=True)
+    if not files:
+        raise SystemExit("no counter_collection CSV under %s" % folder)
+    rows = {}
+    for fn in files:
+        with open(fn, newline="") as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row.get("Kernel_Name", "")
+                for prefix, key in KERNELS.items():
+                    if name.startswith(prefix):
+                        rows.setdefault(key, []).append(
+                            (int(row["Grid_Size"]), float(row["Counter_Value"])))
+    out = {k: [v for _, v in r] for k, r in rows.items()}
+    if "copy_calibration" in rows:
+        g = max(g for g, _ in rows["copy_calibration"])
+        out["copy_calibration"] = [v for gs, v in rows["copy_calibration"] if gs == g]
+    return out
+
+
+def parse(fetch_dir, write_dir, out_path):
+    fetch = _read(fetch_dir, "FETCH_SIZE")
+    write = _read(write_dir, "WRITE_SIZE")
+    if "copy_calibration" not in fetch or "copy_calibration" not in write:
+        raise SystemExit("copy kernel missing from the counter CSVs")
+    kib = COPY_BYTES / 1024.0
+    cf = kib / (sum(fetch["copy_calibration"]) / len(fetch["copy_calibration"]))
+    cw = kib / (sum(write["copy_calibration"]) / len(write["copy_calibration"]))
+    res = {"calibration": {"copy_bytes": COPY_BYTES, "fetch_factor": round(cf, 4),
+                           "write_factor": round(cw, 4)},
+           "kernels": {}}
+    for key in KERNELS.values():
+        if key == "copy_calibration" or key not in fetch or key not in write:
+            continue
+        f = sum(fetch[key]) / len(fetch[key])
+        w = sum(write[key]) / len(write[key])
+        res["kernels"][key] = {
+            "dispatches": len(fetch[key]),
+            "fetch_kib_raw": round(f, 1), "write_kib_raw": round(w, 1),
+            "fetch_bytes": int(f * 1024 * cf), "write_bytes": int(w * 1024 * cw),
+            "hbm_bytes": int(f * 1024 * cf + w * 1024 * cw)}
+    with open(out_path, "w") as fo:
+        json.dump(res, fo, indent=1)
+        fo.write("\n")
+    print(json.dumps(res, indent=1))
+    return 0
+
+
+if __name__ == "__main__":
+    if len(sys.argv) >= 2 and sys.argv[1] == "run":
+        sys.exit(run())
+    if len(sys.argv) == 5 and sys.argv[1] == "parse":
+        sys.exit(parse(sys.argv[2], sys.argv[3], sys.argv[4]))
+    print(__doc__)
+    sys.exit(2)
