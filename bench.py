@@ -4,7 +4,7 @@
 Headline (value): Mrays/s of one 1920x1080 frame of primary rays plus one
 shadow ray per lit hit over the 512^3 heightfield brick map (u16 materials,
 8^3 bricks), scene and outputs resident in HBM.  Beside it in the same JSON
-line: config-3 meshing (4096 chunks of 32^3 u16) batched vs plain, serving
+line: the meshing of 4096 chunks of 32^3 u16) batched vs plain, serving
 (continuous batching over 16 resident scenes at a fixed QPS, p50/p99), the
 2048^3 cave frame loop (1% of chunks edited and remeshed, then a 3840x2160
 frame with shadow and AO rays; alone and under serving load) and the scalar

@@ -17,7 +17,7 @@ def _f32(v):
     return [float(np.float32(x)) for x in v]
 
 
-# ---- config 1: dense 128^3 u8, 8 spheres, 256x256, camera at 1.5x diagonal
+# ---- dense 128^3 u8, 8 spheres, 256x256, camera at 1.5x diagonal
 SPHERES = dict(n=128, seed=1, n_spheres=8, width=256, height=256, vfov=60.0)
 
 
@@ -30,7 +30,7 @@ def spheres_camera(n=128, width=256, height=256, direction=(1.0, 0.8, 0.6)):
                 height=height)
 
 
-# ---- config 2: 512^3 brick map heightfield, 1920x1080 primary + shadow
+# ---- 512^3 brick map heightfield, 1920x1080 primary + shadow
 TERRAIN = dict(n=512, seed=2, width=1920, height=1080, vfov=60.0)
 
 
@@ -40,10 +40,10 @@ def terrain_camera(n=512, width=1920, height=1080):
                 up=[0.0, 1.0, 0.0], vfov=60.0, width=width, height=height)
 
 
-# ---- config 3: 4096 chunks of 32^3 u16
+# ---- 4096 chunks of 32^3 u16
 CHUNKS = dict(n_chunks=4096, seed=3)
 
-# ---- config 4: serving over 16 resident terrain/cave scenes
+# ---- serving over 16 resident terrain/cave scenes
 SERVE_RESOLUTIONS = [(256, 256), (640, 360), (1280, 720), (1920, 1080)]
 
 
@@ -78,7 +78,7 @@ def serve_requests(n_scenes_edges, n_requests, qps, seed=4):
     return reqs
 
 
-# ---- config 5: 2048^3 two-level caves, 3840x2160, primary + shadow + 4 AO
+# ---- 2048^3 two-level caves, 3840x2160, primary + shadow + 4 AO
 CAVES = dict(n=2048, seed=5, width=3840, height=2160, ao_samples=4, ao_radius=16.0)
 
 

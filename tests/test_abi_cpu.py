@@ -57,3 +57,16 @@ def test_struct_sizes_match_header_layout():
     assert C.sizeof(L.RayOutputs) == 56
     assert C.sizeof(L.ChunkSource) == 32
     assert C.sizeof(L.GenerateDesc) == 32
+
+
+def test_engine_instance_builds_and_rejects_bad_arguments():
+    """The headless C++ instance links against the library; without a GPU
+    it fails with the library's message instead of crashing."""
+    import subprocess
+    b = os.path.join(ROOT, "mesoengine_amd", "_build")
+    exe = os.path.join(b, "meso_instance")
+    assert os.path.exists(exe) and os.path.exists(os.path.join(b, "libmeso_engine.so"))
+    p = subprocess.run([exe, "--scene", "nowhere"], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 2 and "usage" in p.stderr
+    eng = C.CDLL(os.path.join(b, "libmeso_engine.so"))
+    assert eng is not None

@@ -30,7 +30,7 @@ def tiles32(grid):
 
 @pytest.mark.parametrize("mode", [L.MESH_CULLED, L.MESH_GREEDY])
 def test_dense_grid_u8_chunks(ctx, mode):
-    # the config-1 128^3 u8 sphere grid meshed as 4^3 chunks of 32^3
+    # the 128^3 u8 sphere grid meshed as 4^3 chunks of 32^3
     g = O.spheres_dense(W.SPHERES["seed"], 128, 8)
     buf = ctx.upload(g.ravel())
     chunks = tiles32(g)

@@ -1,0 +1,106 @@
+"""The headless C++ instance (engine side of the C-ABI) against the oracle.
+
+The driver binary runs the instance loop for a few frames: every frame it
+edits and remeshes a seeded set of dirty chunks (UpdatePhysics) and renders
+primary + shadow + AO rays (Render) while the camera flies through chunk
+borders.  Its dump holds the last frame's outputs, the last remesh and every
+frame's dirty set; the oracle replays the same edits on the same scene and
+must give the same bits.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from mesoengine_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "mesoengine_amd", "_build", "meso_instance")
+
+pytestmark = pytest.mark.gpu
+
+
+def _read_dump(path):
+    b = open(path, "rb").read()
+    assert b[:8] == b"MESOINST"
+    off = 8
+    version, w, h, ib, frames = np.frombuffer(b, np.int32, 5, off)
+    assert version == 1
+    off += 20
+    cam = np.frombuffer(b, np.float32, 10, off)
+    off += 40
+    sets = []
+    for _ in range(frames):
+        seed, nd = np.frombuffer(b, np.uint32, 2, off)
+        off += 8
+        xyz = np.frombuffer(b, np.int32, 3 * int(nd), off).reshape(-1, 3)
+        off += 12 * int(nd)
+        sets.append((int(seed), xyz))
+    nch = int(np.frombuffer(b, np.uint32, 1, off)[0])
+    off += 4
+    counts = np.frombuffer(b, np.uint32, nch, off)
+    off += 4 * nch
+    total = int(np.frombuffer(b, np.uint64, 1, off)[0])
+    off += 8
+    quads = np.frombuffer(b, np.uint32, total, off)
+    off += 4 * total
+    px = int(w) * int(h)
+    idx = np.frombuffer(b, np.int32 if ib == 4 else np.int64, px, off)
+    off += px * int(ib)
+    face = np.frombuffer(b, np.uint8, px, off)
+    off += px
+    t = np.frombuffer(b, np.float32, px, off)
+    off += 4 * px
+    shade = np.frombuffer(b, np.float32, px, off)
+    off += 4 * px
+    assert off == len(b)
+    return dict(w=int(w), h=int(h), cam=cam, sets=sets, counts=counts, quads=quads, idx=idx,
+                face=face, t=t, shade=shade)
+
+
+@pytest.mark.parametrize("scene,gen,n", [("caves", O.GEN_CAVES, 256),
+                                         ("terrain_caves", O.GEN_TERRAIN_CAVES, 200)])
+def test_instance_frames_match_oracle(tmp_path, scene, gen, n):
+    assert os.path.exists(BIN), "engine driver not built"
+    dump = str(tmp_path / "inst.bin")
+    seed = 5
+    cmd = [BIN, "--scene", scene, "--n", str(n), "--seed", str(seed), "--width", "160",
+           "--height", "120", "--frames", "3", "--dirty", "24", "--ao", "4", "--step", "20",
+           "--dump", dump]
+    print("running", " ".join(cmd), flush=True)
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    rep = json.loads(p.stdout.strip().splitlines()[-1])
+    assert rep["frames"] == 3 and all(ms > 0 for ms in rep["render_ms"])
+    assert rep["chunk_updates"] >= 1, "the camera never crossed a chunk border"
+    d = _read_dump(dump)
+
+    slots, masks, vox = O.build_brickmap(gen, n, seed)
+    if scene == "caves":
+        top, slots, masks, vox = O.two_level_from_bricks(n, slots, masks, vox)
+    masks, vox = np.ascontiguousarray(masks.copy()), np.ascontiguousarray(vox.copy())
+    osc = (O.two_level_scene(n, top, slots, masks, vox) if scene == "caves"
+           else O.brickmap_scene(n, slots, masks, vox))
+    for s, xyz in d["sets"]:
+        O.edit_chunks(osc, np.ascontiguousarray(xyz), s)
+
+    # the last frame's remesh
+    last = d["sets"][-1][1]
+    refs = [O.mesh_chunk(O.scene_chunk(osc, *map(int, c)), True) for c in last]
+    assert np.array_equal(d["counts"], [len(r) for r in refs])
+    assert np.array_equal(d["quads"], np.concatenate(refs))
+    assert rep["quads"][-1] == len(d["quads"])
+
+    # the last frame's image
+    c = d["cam"]
+    ocam = O.camera(c[0:3].tolist(), c[3:6].tolist(), c[6:9].tolist(), float(c[9]), d["w"], d["h"])
+    ref = O.raymarch(osc, ocam, O.RAY_SHADOW | O.RAY_AO, O.shading(W.LIGHT, seed, 4, 16.0),
+                     threads=8)
+    assert np.array_equal(d["idx"], ref[0])
+    assert np.array_equal(d["face"], ref[1])
+    assert np.array_equal(d["t"].view(np.uint32), ref[2].view(np.uint32))
+    assert np.array_equal(d["shade"].view(np.uint32), ref[3].view(np.uint32))
+    assert (d["idx"] >= 0).mean() > 0.2, "camera sees too little of the scene"

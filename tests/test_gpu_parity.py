@@ -64,7 +64,7 @@ def test_spheres_raymarch_bitexact(ctx, flags):
     gpu = tg.results()
     osc = O.dense_scene(O.spheres_dense(cfg["seed"], 128, 8))
     ref = O.raymarch(osc, ocam, flags, O.shading(W.LIGHT), threads=8)
-    _assert_rays_equal(gpu, ref, "config-1 dense")
+    _assert_rays_equal(gpu, ref, "128^3 sphere grid")
     cnt = tg.counts()
     rc = ref[4].as_dict()
     for k in ("n_vox", "n_iter", "hits", "rays", "shadow_rays"):
@@ -96,7 +96,7 @@ def test_heightfield_raymarch_full_frame_bitexact(ctx):
     slots, masks, vox = O.build_brickmap(O.GEN_HEIGHTFIELD, cfg["n"], cfg["seed"])
     osc = O.brickmap_scene(cfg["n"], slots, masks, vox)
     ref = O.raymarch(osc, ocam, O.RAY_SHADOW, O.shading(W.LIGHT), threads=16)
-    _assert_rays_equal(gpu, ref, "config-2 brick map")
+    _assert_rays_equal(gpu, ref, "512^3 heightfield brick map")
     cnt = tg.counts()
     rc = ref[4].as_dict()
     for k in ("n_brickslot", "n_brickmask", "n_coarse", "n_iter", "hits", "rays", "shadow_rays"):

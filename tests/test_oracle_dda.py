@@ -237,7 +237,7 @@ def test_axis_parallel_rays_distance():
     assert h.index == -1 and h.face == 0xFF and np.isinf(h.t) and cnt.n_vox == n
 
 
-def test_raymarch_invariants_config1():
+def test_raymarch_invariants_sphere_grid():
     g = O.spheres_dense(1, 128, 8)
     sc = O.dense_scene(g)
     import math
