@@ -52,6 +52,15 @@ def raymarch_leg(ctx, R, L, W, steps, warmup):
     sh = R.shading(W.LIGHT)
     tg = R.RayTargets(ctx, w, h, counters=True)
     flags = L.RAY_SHADOW
+    # the autotuner times the tiled shapes and the persistent waves and keeps
+    # the fastest for this scene / frame size; meso_raymarch then uses it
+    import ctypes as C
+    tune = L.TuneResult()
+    tune.size = C.sizeof(tune)
+    L.check(ctx.lib.meso_autotune_raymarch(ctx.h, sc.h, L.byref(cam), flags, L.byref(sh),
+                                           L.byref(tg.out), L.byref(tune)))
+    log("raymarch tune: variant %d (grid %d, tile %dx%d) %.3f ms, default %.3f ms" % (
+        tune.variant, tune.grid, tune.tile_w, tune.tile_h, tune.best_ms, tune.default_ms))
     # untimed counting pass: the algorithmic byte model needs the step counts
     R.raymarch(ctx, sc, cam, flags | L.RAY_COUNT, sh, tg)
     ctx.sync()
@@ -76,7 +85,9 @@ def raymarch_leg(ctx, R, L, W, steps, warmup):
     achieved = alg_bytes / (ms * 1e-3) / 1e9
     res = dict(ms=ms, wall_ms=wall * 1e3 / steps, rays=rays, primary=cnt["rays"],
                shadow=cnt["shadow_rays"], counts=cnt, alg_bytes=alg_bytes, achieved_gbs=achieved,
-               mrays=rays / (ms * 1e-3) / 1e6, mrays_primary=cnt["rays"] / (ms * 1e-3) / 1e6)
+               mrays=rays / (ms * 1e-3) / 1e6, mrays_primary=cnt["rays"] / (ms * 1e-3) / 1e6,
+               tune=dict(variant=tune.variant, grid=tune.grid, tile=[tune.tile_w, tune.tile_h],
+                         best_ms=round(tune.best_ms, 4), default_ms=round(tune.default_ms, 4)))
     sc.destroy()
     return res
 
@@ -273,6 +284,7 @@ def main():
                      "alg_bytes_per_launch": rm["alg_bytes"]},
         "cpu_baseline": cpu,
         "mrays_primary": round(rm["mrays_primary"], 2),
+        "raymarch_tune": rm["tune"],
         "meshing": mesh,
         "serving": serve,
         "frame": frame,

@@ -637,7 +637,8 @@ meso_status meso_raymarch(meso_context *ctx, meso_scene *scene, const meso_camer
                   cam->height, flags & ~MESO_RAY_COUNT);
     auto it = ctx->tune.find(key);
     if (it != ctx->tune.end()) {
-      tuned = RayLaunch{it->second.variant, it->second.block, it->second.tile_w, it->second.tile_h};
+      tuned = RayLaunch{it->second.variant, it->second.block, it->second.tile_w, it->second.tile_h,
+                        it->second.grid};
       shape = &tuned;
     }
   }

@@ -71,9 +71,11 @@ struct RayOut {
 
 // raymarch launch shape (autotuner variants)
 struct RayLaunch {
-  int32_t variant;  // 0: 8x8 wave packets in a 16x16 tile; 1: 8x8 packets, 32x8 tile; ...
+  int32_t variant;  // 0..4: 8x8 wave packets in 16x16, 32x8, 8x8, 16x8, 32x16 tiles;
+                    // 5: persistent waves pulling 8x8 packets from per-XCD band queues
   int32_t block;
   int32_t tile_w, tile_h;
+  int32_t grid;     // variant 5: workgroups (0 = one per wave slot of the device)
 };
 
 struct MeshLaunch {

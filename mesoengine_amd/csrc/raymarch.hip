@@ -1,6 +1,7 @@
 // Raymarch kernels: one lane per pixel, wave64 packets of 8x8 pixels so the
-// rays of a wave fetch the same bricks; workgroups are remapped so that each
-// XCD (own L2) renders a contiguous band of tiles.
+// rays of a wave fetch the same bricks.  Tiled variants remap workgroups so
+// that each XCD (own L2) renders a contiguous band of tiles; the persistent
+// variant keeps every wave slot busy and pulls packets from per-band queues.
 #include "traverse.h"
 
 namespace meso {
@@ -63,6 +64,91 @@ __global__ void __launch_bounds__(TW *TH) raymarch_kernel(DevScene s, DevCamera 
   }
 }
 
+// Persistent waves: one 64-lane workgroup per wave slot of the device.  The
+// screen's 8x8 packets are cut into 8 bands of rows; the waves of XCD x
+// (workgroup i runs on XCD i % 8) take packets of band x from that band's
+// counter, then help the other bands in turn.  A wave leaves once every
+// counter has passed the end of its band, so the grid always drains; slots
+// freed by short (sky) packets are refilled at once instead of waiting for
+// the next workgroup dispatch.
+template <int KIND, bool COUNT>
+__global__ void __launch_bounds__(64) raymarch_persistent_kernel(
+    DevScene s, DevCamera cam, uint32_t flags, DevShading sh, RayOut out, uint32_t packets_x,
+    uint32_t n_packets, unsigned int *queue) {
+  const uint32_t lane = threadIdx.x;
+  const uint32_t home = blockIdx.x & 7u;
+  StepCounts cnt = {0, 0, 0, 0, 0, 0, 0};
+  uint32_t n_shadow = 0, n_ao = 0, n_hits = 0, n_rays = 0;
+  for (uint32_t k = 0; k < 8u; k++) {
+    const uint32_t band = (home + k) & 7u;
+    const uint32_t lo = (uint32_t)(((uint64_t)n_packets * band) >> 3);
+    const uint32_t hi = (uint32_t)(((uint64_t)n_packets * (band + 1u)) >> 3);
+    for (;;) {
+      uint32_t p = 0;
+      if (lane == 0) p = atomicAdd(&queue[band * 16u], 1u);  // one counter per 64 B line
+      p = lo + (uint32_t)__shfl((int)p, 0, 64);
+      if (p >= hi) break;  // uniform over the wave
+      const int32_t px = (int32_t)((p % packets_x) * 8u + (lane & 7u));
+      const int32_t py = (int32_t)((p / packets_x) * 8u + (lane >> 3));
+      if (px < cam.width && py < cam.height) {
+        const PixelResult r = shade_pixel<KIND, COUNT>(s, cam, flags, sh, px, py, cnt, n_shadow, n_ao);
+        const int64_t pix = (int64_t)py * cam.width + px;
+        if (out.index_bytes == 8)
+          ((int64_t *)out.index)[pix] = r.index;
+        else
+          ((int32_t *)out.index)[pix] = (int32_t)r.index;
+        if (out.face) out.face[pix] = (uint8_t)r.face;
+        if (out.t) out.t[pix] = r.t;
+        if (out.shade) out.shade[pix] = r.shade;
+        n_hits += r.index >= 0 ? 1u : 0u;
+        n_rays++;
+      }
+    }
+  }
+  if constexpr (COUNT) {
+    unsigned long long v[MESO_CNT_COUNT] = {cnt.n_vox, cnt.n_top, cnt.n_node, cnt.n_brickslot,
+                                            cnt.n_brickmask, n_hits, n_rays, n_shadow, n_ao,
+                                            cnt.n_coarse, cnt.n_iter};
+#pragma unroll
+    for (int c = 0; c < MESO_CNT_COUNT; c++) {
+      unsigned long long x = v[c];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+      if (lane == 0 && out.counters && x) atomicAdd(&out.counters[c], x);
+    }
+  }
+}
+
+template <int KIND, bool COUNT>
+static meso_status launch_persistent(meso_context *ctx, const meso_scene *scene,
+                                     const DevCamera &cam, uint32_t flags, const DevShading &sh,
+                                     const RayOut &out, hipStream_t stream, int grid) {
+  const uint32_t packets_x = (uint32_t)((cam.width + 7) / 8);
+  const uint32_t n_packets = packets_x * (uint32_t)((cam.height + 7) / 8);
+  if (grid <= 0) {
+    int per_cu = 0;
+    MESO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, raymarch_persistent_kernel<KIND, COUNT>, 64, 0));
+    grid = (per_cu < 1 ? 1 : per_cu) * ctx->props.multiProcessorCount;
+  }
+  grid = (grid + 7) & ~7;  // every XCD gets the same number of waves
+  // the band counters live for this launch only (stream-ordered), so
+  // launches on different streams never share them
+  unsigned int *queue = nullptr;
+  MESO_HIP(hipMallocAsync((void **)&queue, 8 * 64, stream));
+  meso_status st = launch_zero(queue, 8 * 64, stream);
+  if (st == MESO_OK) {
+    hipLaunchKernelGGL((raymarch_persistent_kernel<KIND, COUNT>), dim3((uint32_t)grid), dim3(64), 0,
+                       stream, scene->dev(), cam, flags, sh, out, packets_x, n_packets, queue);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) st = fail(MESO_ERR_RUNTIME, "raymarch launch: %s", hipGetErrorString(e));
+  }
+  const hipError_t ef = hipFreeAsync(queue, stream);
+  if (st == MESO_OK && ef != hipSuccess)
+    st = fail(MESO_ERR_RUNTIME, "hipFreeAsync: %s", hipGetErrorString(ef));
+  return st;
+}
+
 template <int KIND, bool COUNT, int TW, int TH>
 static meso_status launch_shape(const meso_scene *scene, const DevCamera &cam, uint32_t flags,
                                 const DevShading &sh, const RayOut &out, hipStream_t stream) {
@@ -76,36 +162,38 @@ static meso_status launch_shape(const meso_scene *scene, const DevCamera &cam, u
 }
 
 template <int KIND, bool COUNT>
-static meso_status launch_kind(const meso_scene *scene, const DevCamera &cam, uint32_t flags,
-                               const DevShading &sh, const RayOut &out, hipStream_t stream,
-                               int variant) {
+static meso_status launch_kind(meso_context *ctx, const meso_scene *scene, const DevCamera &cam,
+                               uint32_t flags, const DevShading &sh, const RayOut &out,
+                               hipStream_t stream, int variant, int grid) {
   switch (variant) {
     case 0: return launch_shape<KIND, COUNT, 16, 16>(scene, cam, flags, sh, out, stream);
     case 1: return launch_shape<KIND, COUNT, 32, 8>(scene, cam, flags, sh, out, stream);
     case 2: return launch_shape<KIND, COUNT, 8, 8>(scene, cam, flags, sh, out, stream);
     case 3: return launch_shape<KIND, COUNT, 16, 8>(scene, cam, flags, sh, out, stream);
     case 4: return launch_shape<KIND, COUNT, 32, 16>(scene, cam, flags, sh, out, stream);
+    case 5: return launch_persistent<KIND, COUNT>(ctx, scene, cam, flags, sh, out, stream, grid);
     default: return fail(MESO_ERR_ARGUMENT_OUT_OF_RANGE, "unknown raymarch variant %d", variant);
   }
 }
 
-RayLaunch default_ray_launch(const meso_scene *) { return RayLaunch{0, 256, 16, 16}; }
+RayLaunch default_ray_launch(const meso_scene *) { return RayLaunch{0, 256, 16, 16, 0}; }
 
-meso_status launch_raymarch(meso_context *, const meso_scene *scene, const DevCamera &cam,
+meso_status launch_raymarch(meso_context *ctx, const meso_scene *scene, const DevCamera &cam,
                             uint32_t flags, const DevShading &sh, const RayOut &out,
                             hipStream_t stream, const RayLaunch *shape) {
   const int variant = shape ? shape->variant : 0;
+  const int grid = shape ? shape->grid : 0;
   const bool count = (flags & MESO_RAY_COUNT) != 0;
   switch (scene->kind) {
     case MESO_SCENE_DENSE_U8:
-      return count ? launch_kind<1, true>(scene, cam, flags, sh, out, stream, variant)
-                   : launch_kind<1, false>(scene, cam, flags, sh, out, stream, variant);
+      return count ? launch_kind<1, true>(ctx, scene, cam, flags, sh, out, stream, variant, grid)
+                   : launch_kind<1, false>(ctx, scene, cam, flags, sh, out, stream, variant, grid);
     case MESO_SCENE_BRICKMAP_U16:
-      return count ? launch_kind<2, true>(scene, cam, flags, sh, out, stream, variant)
-                   : launch_kind<2, false>(scene, cam, flags, sh, out, stream, variant);
+      return count ? launch_kind<2, true>(ctx, scene, cam, flags, sh, out, stream, variant, grid)
+                   : launch_kind<2, false>(ctx, scene, cam, flags, sh, out, stream, variant, grid);
     case MESO_SCENE_TWO_LEVEL_U16:
-      return count ? launch_kind<3, true>(scene, cam, flags, sh, out, stream, variant)
-                   : launch_kind<3, false>(scene, cam, flags, sh, out, stream, variant);
+      return count ? launch_kind<3, true>(ctx, scene, cam, flags, sh, out, stream, variant, grid)
+                   : launch_kind<3, false>(ctx, scene, cam, flags, sh, out, stream, variant, grid);
     default: return fail(MESO_ERR_ARGUMENT_OUT_OF_RANGE, "unknown scene kind %d", scene->kind);
   }
 }

@@ -192,15 +192,19 @@ meso_status meso_autotune_raymarch(meso_context *ctx, meso_scene *scene, const m
       (shading->light[0] != 0.0f || shading->light[1] != 0.0f || shading->light[2] != 0.0f))
     derive_shading(shading, &sh);
   RayOut ro{outp->hit_index, outp->index_bytes, outp->face, outp->t, outp->shade, nullptr};
+  // tiled shapes, then persistent waves at full occupancy (grid 0) and at
+  // 24 / 16 waves per CU
+  const int cus = ctx->props.multiProcessorCount;
   struct Shape {
-    int v, tw, th;
-  } shapes[] = {{0, 16, 16}, {1, 32, 8}, {2, 8, 8}, {3, 16, 8}, {4, 32, 16}};
+    int v, tw, th, grid;
+  } shapes[] = {{0, 16, 16, 0}, {1, 32, 8, 0},         {2, 8, 8, 0},         {3, 16, 8, 0},
+                {4, 32, 16, 0}, {5, 8, 8, 0},          {5, 8, 8, 24 * cus},  {5, 8, 8, 16 * cus}};
   meso_tune_result best{};
   best.size = sizeof(best);
   best.best_ms = 1e30f;
   int tried = 0;
   for (const Shape &sp : shapes) {
-    RayLaunch rl{sp.v, sp.tw * sp.th, sp.tw, sp.th};
+    RayLaunch rl{sp.v, sp.tw * sp.th, sp.tw, sp.th, sp.grid};
     const float ms = time_ms(
         ctx, [&]() { return launch_raymarch(ctx, scene, dc, flags, sh, ro, ctx->stream, &rl); }, 5,
         &st);
@@ -210,7 +214,8 @@ meso_status meso_autotune_raymarch(meso_context *ctx, meso_scene *scene, const m
     if (ms < best.best_ms) {
       best.best_ms = ms;
       best.variant = sp.v;
-      best.block = sp.tw * sp.th;
+      best.block = sp.v == 5 ? 64 : sp.tw * sp.th;
+      best.grid = sp.grid;
       best.tile_w = sp.tw;
       best.tile_h = sp.th;
     }
@@ -256,7 +261,7 @@ meso_status meso_autotune_load(meso_context *ctx, const char *path) {
     if (!(is >> key >> r.variant >> r.block >> r.grid >> r.tile_w >> r.tile_h >> r.best_ms >>
           r.default_ms))
       continue;
-    if (r.variant < 0 || r.variant > 4 || r.grid < 0) continue;
+    if (r.variant < 0 || r.variant > 5 || r.grid < 0) continue;
     ctx->tune[key] = r;
   }
   return MESO_OK;

@@ -68,6 +68,17 @@ class _Frames:
         self.mt = R.MeshTargets(ctx, self.n_dirty, 24 << 20)
         self.ev = [R.Event(ctx) for _ in range(4)]
 
+    def tune(self):
+        """autotune the frame's launch shape (kept by the library for this key)"""
+        import ctypes as C
+        t = L.TuneResult()
+        t.size = C.sizeof(t)
+        L.check(self.ctx.lib.meso_autotune_raymarch(self.ctx.h, self.scene.h, L.byref(self.cam),
+                                                    self.flags, L.byref(self.sh),
+                                                    L.byref(self.tg.out), L.byref(t)))
+        return dict(variant=t.variant, grid=t.grid, tile=[t.tile_w, t.tile_h],
+                    best_ms=round(t.best_ms, 4), default_ms=round(t.default_ms, 4))
+
     def count(self):
         R.raymarch(self.ctx, self.scene, self.cam, self.flags | L.RAY_COUNT, self.sh, self.tg)
         self.ctx.sync()
@@ -114,6 +125,8 @@ def run(ctx, frames=8, warmup=2, serve_qps=1000.0, serve_requests=400, budget_by
     cfg = W.CAVES
     fr = _Frames(ctx, cfg, seed, log)
     try:
+        tune = fr.tune()
+        log("frame render tune: %s" % tune)
         cnt = fr.count()
         rays = cnt["rays"] + cnt["shadow_rays"] + cnt["ao_rays"]
         alg = frame_bytes(cnt)
@@ -172,6 +185,7 @@ def run(ctx, frames=8, warmup=2, serve_qps=1000.0, serve_requests=400, budget_by
                              "3840x2160 primary + shadow + 4 AO rays per hit; 1% of 32^3 chunks "
                              "edited and greedy-remeshed per frame",
                     scene=fr.info, dirty_chunks=fr.n_dirty, counts=cnt, rays_per_frame=rays,
+                    render_tune=tune,
                     render_alg_bytes=alg, alone=alone, under_serving_load=loaded)
     finally:
         fr.close()

@@ -18,6 +18,10 @@ Usage (rocprofv3 run from /tmp, the program straight after `--`):
   rocprofv3 --pmc WRITE_SIZE -d OUT/write -o run --output-format csv -- \
       python3 REPO/mesoengine_amd/traffic.py run
   python3 REPO/mesoengine_amd/traffic.py parse OUT/fetch OUT/write REPO/profiles/traffic.json
+
+Other counter passes of the same workload (L2 hits, occupancy, VALU, LDS) are
+summarised per kernel with
+  python3 REPO/mesoengine_amd/traffic.py parse-counters OUT.json DIR [DIR ...]
 """
 import csv
 import glob
@@ -145,10 +149,66 @@ def parse(fetch_dir, write_dir, out_path):
     return 0
 
 
+CU_NUM = 256
+XCC_NUM = 8  # GRBM_GUI_ACTIVE comes summed over the 8 XCDs
+
+
+def _read_all(folder):
+    """kernel key -> counter -> per-dispatch values, for every counter in the CSVs"""
+    files = glob.glob(os.path.join(folder, "**", "*counter_collection*.csv"), recursive=True)
+    out = {}
+    for fn in files:
+        with open(fn, newline="") as f:
+            for row in csv.DictReader(f):
+                name = row.get("Kernel_Name", "")
+                for prefix, key in KERNELS.items():
+                    if key != "copy_calibration" and name.startswith(prefix):
+                        out.setdefault(key, {}).setdefault(row["Counter_Name"], []).append(
+                            float(row["Counter_Value"]))
+    return out
+
+
+def parse_counters(out_path, folders):
+    """Per-kernel averages of the counters of several passes, plus the derived
+    rates: L2 hit rate, L1 (TCP) hit rate, mean resident waves per CU (of
+    32), VALU busy (share of SIMD cycles issuing VALU work; the SQ counters
+    are in quad-cycles), VALU lane utilisation, LDS bank-conflict ratio."""
+    merged = {}
+    for folder in folders:
+        for key, counters in _read_all(folder).items():
+            for c, vals in counters.items():
+                merged.setdefault(key, {})["%s" % c] = sum(vals) / len(vals)
+    res = {}
+    for key, c in merged.items():
+        d = {k: round(v, 1) for k, v in sorted(c.items())}
+        g = c.get("GRBM_GUI_ACTIVE", 0.0) / XCC_NUM  # the kernel's active cycles
+        if "TCC_HIT_sum" in c and "TCC_MISS_sum" in c:
+            d["l2_hit_rate"] = round(c["TCC_HIT_sum"] / max(1.0, c["TCC_HIT_sum"] + c["TCC_MISS_sum"]), 4)
+        if g and "SQ_WAVE_CYCLES" in c:
+            d["mean_waves_per_cu"] = round(4 * c["SQ_WAVE_CYCLES"] / g / CU_NUM, 2)
+        if g and "SQ_ACTIVE_INST_VALU" in c:
+            d["valu_busy_pct"] = round(100 * c["SQ_ACTIVE_INST_VALU"] / CU_NUM / g, 1)
+        if "SQ_THREAD_CYCLES_VALU" in c and c.get("SQ_ACTIVE_INST_VALU"):
+            d["valu_lane_util_pct"] = round(100 * c["SQ_THREAD_CYCLES_VALU"] / (c["SQ_ACTIVE_INST_VALU"] * 64), 1)
+        if c.get("TCP_TOTAL_CACHE_ACCESSES_sum") and "TCP_TCC_READ_REQ_sum" in c:
+            d["l1_hit_rate"] = round(1 - c["TCP_TCC_READ_REQ_sum"] / c["TCP_TOTAL_CACHE_ACCESSES_sum"], 4)
+        if "SQ_LDS_BANK_CONFLICT" in c and c.get("SQ_LDS_IDX_ACTIVE"):
+            d["lds_bank_conflict_ratio"] = round(
+                c["SQ_LDS_BANK_CONFLICT"] / max(1.0, c["SQ_LDS_IDX_ACTIVE"] - c["SQ_LDS_BANK_CONFLICT"]), 4)
+        res[key] = d
+    with open(out_path, "w") as fo:
+        json.dump(res, fo, indent=1)
+        fo.write("\n")
+    print(json.dumps(res, indent=1))
+    return 0
+
+
 if __name__ == "__main__":
     if len(sys.argv) >= 2 and sys.argv[1] == "run":
         sys.exit(run())
     if len(sys.argv) == 5 and sys.argv[1] == "parse":
         sys.exit(parse(sys.argv[2], sys.argv[3], sys.argv[4]))
+    if len(sys.argv) >= 4 and sys.argv[1] == "parse-counters":
+        sys.exit(parse_counters(sys.argv[2], sys.argv[3:]))
     print(__doc__)
     sys.exit(2)

@@ -1,0 +1,93 @@
+"""Every raymarch launch shape gives the oracle's bits and step counts.
+
+The five tiled shapes and the persistent waves (variant 5, at the device's
+full occupancy and at a small grid that makes every wave serve several
+bands) are forced through the autotuner's cache file, then run through
+meso_raymarch with the step counters on.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from mesoengine_amd import _lib as L
+from mesoengine_amd import runtime as R
+from mesoengine_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+# (variant, tile_w, tile_h, grid)
+SHAPES = [(0, 16, 16, 0), (1, 32, 8, 0), (2, 8, 8, 0), (3, 16, 8, 0), (4, 32, 16, 0),
+          (5, 8, 8, 0), (5, 8, 8, 64)]
+
+
+def _force(ctx, tmp_path, kind, n, w, h, flags, shape):
+    v, tw, th, grid = shape
+    key = "ray:%d:%d:%dx%d:%d" % (kind, n, w, h, flags)
+    p = tmp_path / "tune.txt"
+    p.write_text("%s %d %d %d %d %d 1 1\n" % (key, v, 64 if v == 5 else tw * th, grid, tw, th))
+    L.check(ctx.lib.meso_autotune_load(ctx.h, str(p).encode()))
+
+
+def _check(ctx, tg, ref, what):
+    idx, face, t, shade = tg.results()
+    assert np.array_equal(idx, ref[0]), what
+    assert np.array_equal(face, ref[1]), what
+    assert np.array_equal(t.view(np.uint32), ref[2].view(np.uint32)), what
+    assert np.array_equal(shade.view(np.uint32), ref[3].view(np.uint32)), what
+    cnt = tg.counts()
+    rc = ref[4].as_dict()
+    for k in rc:
+        assert cnt[k] == rc[k], (what, k, cnt[k], rc[k])
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_brickmap_shapes_match_oracle(ctx, tmp_path, shape):
+    n, w, h = 256, 328, 200  # frame size used by no other test: its cache key is this test's
+    sc = ctx.scene_generate(L.GEN_HEIGHTFIELD, L.SCENE_BRICKMAP_U16, n, 2)
+    cp = W.terrain_camera(n, w, h)
+    cam = R.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], w, h)
+    _force(ctx, tmp_path, L.SCENE_BRICKMAP_U16, n, w, h, L.RAY_SHADOW, shape)
+    tg = R.RayTargets(ctx, w, h, counters=True)
+    R.raymarch(ctx, sc, cam, L.RAY_SHADOW | L.RAY_COUNT, R.shading(W.LIGHT), tg)
+    ctx.sync()
+    slots, masks, vox = O.build_brickmap(O.GEN_HEIGHTFIELD, n, 2)
+    osc = O.brickmap_scene(n, slots, masks, vox)
+    ocam = O.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], w, h)
+    ref = O.raymarch(osc, ocam, O.RAY_SHADOW, O.shading(W.LIGHT), threads=8)
+    _check(ctx, tg, ref, "variant %s" % (shape,))
+    sc.destroy()
+
+
+@pytest.mark.parametrize("shape", [(5, 8, 8, 0), (5, 8, 8, 24)])
+def test_persistent_two_level_ao_and_dense(ctx, tmp_path, shape):
+    n, w, h = 256, 136, 72
+    sc = ctx.scene_generate(L.GEN_CAVES, L.SCENE_TWO_LEVEL_U16, n, 5)
+    cp = W.caves_camera(n, w, h)
+    cam = R.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], w, h)
+    flags = L.RAY_SHADOW | L.RAY_AO
+    _force(ctx, tmp_path, L.SCENE_TWO_LEVEL_U16, n, w, h, flags, shape)
+    tg = R.RayTargets(ctx, w, h, counters=True)
+    R.raymarch(ctx, sc, cam, flags | L.RAY_COUNT, R.shading(W.LIGHT, 9, 4, 16.0), tg)
+    ctx.sync()
+    slots, masks, vox = O.build_brickmap(O.GEN_CAVES, n, 5)
+    top, ns, m2, v2 = O.two_level_from_bricks(n, slots, masks, vox)
+    osc = O.two_level_scene(n, top, ns, m2, v2)
+    ocam = O.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], w, h)
+    ref = O.raymarch(osc, ocam, flags, O.shading(W.LIGHT, 9, 4, 16.0), threads=8)
+    _check(ctx, tg, ref, "two-level %s" % (shape,))
+    sc.destroy()
+
+    # the dense sphere grid, an odd frame size (partial packets at both edges)
+    w, h = 203, 117
+    sc = ctx.scene_generate(L.GEN_SPHERES, L.SCENE_DENSE_U8, 128, 1, 8)
+    cp = W.spheres_camera(width=w, height=h)
+    cam = R.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], w, h)
+    _force(ctx, tmp_path, L.SCENE_DENSE_U8, 128, w, h, L.RAY_SHADOW, shape)
+    tg = R.RayTargets(ctx, w, h, counters=True)
+    R.raymarch(ctx, sc, cam, L.RAY_SHADOW | L.RAY_COUNT, R.shading(W.LIGHT), tg)
+    ctx.sync()
+    osc = O.dense_scene(O.spheres_dense(1, 128, 8).ravel())
+    ocam = O.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], w, h)
+    ref = O.raymarch(osc, ocam, O.RAY_SHADOW, O.shading(W.LIGHT), threads=8)
+    _check(ctx, tg, ref, "dense %s" % (shape,))
+    sc.destroy()
