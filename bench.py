@@ -205,10 +205,63 @@ def cpu_leg(W, seconds_cap=30.0):
     t = time.perf_counter()
     _, _, _, _, c1 = O.raymarch(sc, cam, O.RAY_SHADOW, sh, row0=508, row1=572, threads=1)
     st = time.perf_counter() - t
-    return dict(value=rays / med / 1e6, unit="Mrays/s", cores=cores, kind="port",
-                sample="full 1920x1080 frame (primary + shadow) of the 512^3 heightfield brick map, "
-                       "median of %d runs on %d threads; scene build %.2fs" % (len(times), cores, gen_s),
-                single_thread_mrays=(c1.rays + c1.shadow_rays) / st / 1e6)
+    out = dict(value=rays / med / 1e6, unit="Mrays/s", cores=cores, kind="port",
+               sample="full 1920x1080 frame (primary + shadow) of the 512^3 heightfield brick map, "
+                      "median of %d runs on %d threads; scene build %.2fs" % (len(times), cores, gen_s),
+               single_thread_mrays=(c1.rays + c1.shadow_rays) / st / 1e6)
+    out.update(cpu_extras(O, cores))
+    return out
+
+
+def _median_time(fn, runs=5, cap_s=6.0):
+    ts = []
+    t0 = time.time()
+    while len(ts) < runs and (time.time() - t0) < cap_s:
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return sorted(ts)[len(ts) // 2]
+
+
+def cpu_extras(O, cores):
+    """The rest of the CPU path on the same host: host facts, the scalar scene
+    generation, the reference's own per-chunk generators (restated) and the
+    scalar culled / greedy mesher on a sample of the 4096-chunk batch."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    host = dict(hardware_concurrency=os.cpu_count(), affinity=affinity, threads_used=cores,
+                model=model)
+    # reference generators per 16^3 chunk (GeneratorHelper.h:90-150), restated in C:
+    # chunks of the 8^3-chunk block around the reference sphere at (100, 0, 0), r 50
+    coords = [(x, y, z) for x in range(3, 11) for y in range(-4, 4) for z in range(-4, 4)]
+    t_sph = _median_time(lambda: [O.ref_generate_sphere(*c) for c in coords], runs=3)
+    coords_t = coords[:64]
+    t_tg = _median_time(lambda: [O.ref_test_generator(*c) for c in coords_t], runs=3)
+    gen = dict(sphere_mvox_s=round(len(coords) * 4096 / t_sph / 1e6, 2),
+               test_generator_mvox_s=round(len(coords_t) * 4096 / t_tg / 1e6, 3),
+               sample="%d sphere chunks, %d TestGenerator chunks of 16^3, 1 thread" % (
+                   len(coords), len(coords_t)))
+    # scalar meshing of 512 chunks of the seeded batch, 1 thread and all threads
+    vox, _ = O.gen_chunks(3, 512)
+    mesh = {}
+    for name, greedy in (("culled", False), ("greedy", True)):
+        t1 = _median_time(lambda: O.mesh_chunks_counts(vox[:64], greedy, 1), runs=3)
+        tn = _median_time(lambda: O.mesh_chunks_counts(vox, greedy, cores), runs=3)
+        mesh[name] = dict(mvox_s_1_thread=round(64 * 32768 / t1 / 1e6, 1),
+                          mvox_s_all_threads=round(512 * 32768 / tn / 1e6, 1))
+    mesh["sample"] = "512 chunks of the 4096-chunk batch (64 on 1 thread), %d threads" % cores
+    return dict(host=host, reference_generators=gen, mesh=mesh)
 
 
 def main():

@@ -328,7 +328,7 @@ meso_status meso_generate_sphere_chunks(meso_context *ctx, const int32_t *d_chun
 
 /* ------------------------------------------------------------------ */
 /* autotuner: times tile / packet shapes on this device and keeps the     */
-/* fastest per workload key (in memory and in an optional JSON cache)    */
+/* fastest per workload key (in memory and in an optional text cache)    */
 /* ------------------------------------------------------------------ */
 typedef struct {
   uint32_t size;
@@ -347,7 +347,17 @@ meso_status meso_autotune_mesh(meso_context *ctx, const meso_chunk_source *src, 
 meso_status meso_autotune_raymarch(meso_context *ctx, meso_scene *scene, const meso_camera *cam,
                                    uint32_t flags, const meso_shading *shading,
                                    const meso_ray_outputs *out, meso_tune_result *result);
-/* Load / save the tuning cache (JSON lines).  Missing file is not an error. */
+/* Pins a raymarch launch shape for the workload key (scene kind and edge,
+ * frame size, flags without MESO_RAY_COUNT), replacing the tuner's choice:
+ * shape->variant 0..4 (tiles 16x16, 32x8, 8x8, 16x8, 32x16; tile_w/tile_h
+ * must match) or 5 (persistent waves; shape->grid workgroups of 64 lanes,
+ * 0 = one per wave slot, at most 32 per CU).  A caller that shares the
+ * device with latency-bound work pins a smaller persistent grid so that
+ * the other work keeps wave slots.  No lvk counterpart. */
+meso_status meso_autotune_pin_raymarch(meso_context *ctx, meso_scene *scene, const meso_camera *cam,
+                                       uint32_t flags, const meso_tune_result *shape);
+/* Load / save the tuning cache: text lines "key variant block grid tile_w
+ * tile_h best_ms default_ms".  A missing file is not an error. */
 meso_status meso_autotune_load(meso_context *ctx, const char *path);
 meso_status meso_autotune_save(meso_context *ctx, const char *path);
 

@@ -141,6 +141,7 @@ _SIGS = {
                                                 C.c_int32, P, P]),
     "meso_autotune_mesh": (C.c_int32, [P, P, C.c_uint32, P, P, P, C.c_uint64, P, P, P]),
     "meso_autotune_raymarch": (C.c_int32, [P, P, P, C.c_uint32, P, P, P]),
+    "meso_autotune_pin_raymarch": (C.c_int32, [P, P, P, C.c_uint32, P]),
     "meso_autotune_load": (C.c_int32, [P, C.c_char_p]),
     "meso_autotune_save": (C.c_int32, [P, C.c_char_p]),
     "meso_serve_start": (C.c_int32, [P, P, PP]),

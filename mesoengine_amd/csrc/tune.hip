@@ -232,6 +232,36 @@ meso_status meso_autotune_raymarch(meso_context *ctx, meso_scene *scene, const m
   return MESO_OK;
 }
 
+meso_status meso_autotune_pin_raymarch(meso_context *ctx, meso_scene *scene, const meso_camera *cam,
+                                       uint32_t flags, const meso_tune_result *shape) {
+  MESO_CHECK(ctx && scene && cam && shape, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  MESO_CHECK(cam->size >= sizeof(meso_camera) && shape->size >= sizeof(meso_tune_result),
+             MESO_ERR_ARGUMENT_OUT_OF_RANGE, "struct size too small");
+  static const int tiles[5][2] = {{16, 16}, {32, 8}, {8, 8}, {16, 8}, {32, 16}};
+  meso_tune_result r = *shape;
+  if (r.variant >= 0 && r.variant <= 4) {
+    MESO_CHECK(r.tile_w == tiles[r.variant][0] && r.tile_h == tiles[r.variant][1],
+               MESO_ERR_ARGUMENT_OUT_OF_RANGE, "variant %d has %dx%d tiles", r.variant,
+               tiles[r.variant][0], tiles[r.variant][1]);
+    r.block = r.tile_w * r.tile_h;
+    r.grid = 0;
+  } else {
+    MESO_CHECK(r.variant == 5, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "unknown raymarch variant %d",
+               r.variant);
+    MESO_CHECK(r.grid >= 0 && r.grid <= 32 * ctx->props.multiProcessorCount,
+               MESO_ERR_ARGUMENT_OUT_OF_RANGE, "persistent grid %d outside 0..%d", r.grid,
+               32 * ctx->props.multiProcessorCount);
+    r.block = 64;
+    r.tile_w = r.tile_h = 8;
+  }
+  char key[128];
+  snprintf(key, sizeof(key), "ray:%d:%d:%dx%d:%u", scene->kind, scene->n, cam->width, cam->height,
+           flags & ~MESO_RAY_COUNT);
+  std::lock_guard<std::mutex> lk(ctx->tune_mu);
+  ctx->tune[key] = r;
+  return MESO_OK;
+}
+
 // one line per entry: key variant block grid tile_w tile_h best_ms default_ms
 meso_status meso_autotune_save(meso_context *ctx, const char *path) {
   MESO_CHECK(ctx && path, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
@@ -261,7 +291,9 @@ meso_status meso_autotune_load(meso_context *ctx, const char *path) {
     if (!(is >> key >> r.variant >> r.block >> r.grid >> r.tile_w >> r.tile_h >> r.best_ms >>
           r.default_ms))
       continue;
-    if (r.variant < 0 || r.variant > 5 || r.grid < 0) continue;
+    if (r.variant < 0 || r.variant > 5 || r.grid < 0 ||
+        r.grid > 32 * ctx->props.multiProcessorCount)
+      continue;
     ctx->tune[key] = r;
   }
   return MESO_OK;

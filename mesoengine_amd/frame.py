@@ -79,6 +79,20 @@ class _Frames:
         return dict(variant=t.variant, grid=t.grid, tile=[t.tile_w, t.tile_h],
                     best_ms=round(t.best_ms, 4), default_ms=round(t.default_ms, 4))
 
+    def pin(self, variant, waves_per_cu=0):
+        """pin the render's launch shape: a tiled variant (0..4), or persistent
+        waves (5) on `waves_per_cu` of the 32 wave slots of each CU"""
+        import ctypes as C
+        t = L.TuneResult()
+        t.size = C.sizeof(t)
+        tiles = [(16, 16), (32, 8), (8, 8), (16, 8), (32, 16), (8, 8)]
+        t.variant = variant
+        t.tile_w, t.tile_h = tiles[variant]
+        t.grid = waves_per_cu * self.ctx.device_info().compute_units if variant == 5 else 0
+        L.check(self.ctx.lib.meso_autotune_pin_raymarch(self.ctx.h, self.scene.h,
+                                                        L.byref(self.cam), self.flags,
+                                                        L.byref(t)))
+
     def count(self):
         R.raymarch(self.ctx, self.scene, self.cam, self.flags | L.RAY_COUNT, self.sh, self.tg)
         self.ctx.sync()
@@ -120,6 +134,45 @@ def _summary(rows):
                 quads_per_frame=int(np.median([r[3] for r in rows])))
 
 
+def _under_load(ctx, fr, scenes, reqs, budget_bytes, frames, rays, log, label):
+    """frames rendered while the serving stream runs; frame split plus the
+    served latency"""
+    sh = R.shading(W.LIGHT)
+    result = {}
+
+    def serve():
+        try:
+            result["lat"], result["makespan"], result["stats"] = serving.served_run(
+                ctx, scenes, reqs, sh, budget_bytes, True)
+        except Exception as ex:  # surfaced after join
+            result["error"] = ex
+
+    th = threading.Thread(target=serve)
+    th.start()
+    rows = []
+    f = 0
+    while th.is_alive() or len(rows) < frames:
+        rows += fr.run(1, first=20_000 + f)
+        f += 1
+        if len(rows) >= 4 * frames and not th.is_alive():
+            break
+    th.join()
+    if "error" in result:
+        raise result["error"]
+    out = _summary(rows)
+    out["render_mrays"] = round(rays / (out["render_ms"] * 1e-3) / 1e6, 1)
+    lat = result["lat"]
+    out["serving"] = dict(p50_ms=round(float(np.percentile(lat, 50) * 1e3), 3),
+                          p99_ms=round(float(np.percentile(lat, 99) * 1e3), 3),
+                          launches=int(result["stats"]["launches"]))
+    out["render_shape"] = label
+    log("frame under serving load, %s: edit %.3f ms, remesh %.3f ms, render %.3f ms, frame "
+        "%.3f ms (max %.3f); served p50 %.2f p99 %.2f ms" % (
+            label, out["edit_ms"], out["remesh_ms"], out["render_ms"], out["frame_ms"],
+            out["frame_ms_max"], out["serving"]["p50_ms"], out["serving"]["p99_ms"]))
+    return out
+
+
 def run(ctx, frames=8, warmup=2, serve_qps=1000.0, serve_requests=400, budget_bytes=512 << 20,
         seed=5, log=print):
     cfg = W.CAVES
@@ -141,46 +194,26 @@ def run(ctx, frames=8, warmup=2, serve_qps=1000.0, serve_requests=400, budget_by
             "frame %.3f ms" % (alone["edit_ms"], alone["remesh_ms"], alone["render_ms"],
                                alone["render_mrays"], alone["frame_ms"]))
 
-        # the same frames while the serving stream runs on the server's streams
+        # the same frames while the serving stream runs on the server's streams,
+        # with three render shapes: the one tuned for the frame alone
+        # (persistent waves on every slot), 16x16 tiles (slots return to the
+        # dispatcher as tiles finish), persistent waves on half of the slots
         edges = W.serve_scenes(4)
         scenes = serving.make_scenes(ctx, edges, 4)
         reqs = W.serve_requests(edges, serve_requests, serve_qps, 4)
-        sh = R.shading(W.LIGHT)
-        result = {}
-
-        def serve():
-            try:
-                result["lat"], result["makespan"], result["stats"] = serving.served_run(
-                    ctx, scenes, reqs, sh, budget_bytes, True)
-            except Exception as ex:  # surfaced after join
-                result["error"] = ex
-
-        th = threading.Thread(target=serve)
-        th.start()
-        rows = []
-        f = 0
-        while th.is_alive() or len(rows) < frames:
-            rows += fr.run(1, first=20_000 + f)
-            f += 1
-            if len(rows) >= 4 * frames and not th.is_alive():
-                break
-        th.join()
-        if "error" in result:
-            raise result["error"]
-        for s in scenes:
-            s.destroy()
-        loaded = _summary(rows)
-        loaded["render_mrays"] = round(rays / (loaded["render_ms"] * 1e-3) / 1e6, 1)
-        lat = result["lat"]
-        loaded["serving"] = dict(qps=serve_qps, requests=serve_requests,
-                                 p50_ms=round(float(np.percentile(lat, 50) * 1e3), 3),
-                                 p99_ms=round(float(np.percentile(lat, 99) * 1e3), 3),
-                                 launches=int(result["stats"]["launches"]))
-        log("frame under serving load (%.0f QPS): edit %.3f ms, remesh %.3f ms, render %.3f ms, "
-            "frame %.3f ms (max %.3f); served p50 %.2f p99 %.2f ms" % (
-                serve_qps, loaded["edit_ms"], loaded["remesh_ms"], loaded["render_ms"],
-                loaded["frame_ms"], loaded["frame_ms_max"], loaded["serving"]["p50_ms"],
-                loaded["serving"]["p99_ms"]))
+        loaded = {}
+        try:
+            for label, pin in (("tuned", None), ("tiled_16x16", (0, 0)),
+                               ("persistent_16_of_32_slots", (5, 16))):
+                if pin:
+                    fr.pin(*pin)
+                loaded[label] = _under_load(ctx, fr, scenes, reqs, budget_bytes, frames, rays,
+                                            log, label)
+                loaded[label]["serving"]["qps"] = serve_qps
+                loaded[label]["serving"]["requests"] = serve_requests
+        finally:
+            for s in scenes:
+                s.destroy()
         return dict(workload="2048^3 two-level caves (u16, 8^3 bricks under 16^3-brick nodes), "
                              "3840x2160 primary + shadow + 4 AO rays per hit; 1% of 32^3 chunks "
                              "edited and greedy-remeshed per frame",

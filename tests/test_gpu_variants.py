@@ -91,3 +91,28 @@ def test_persistent_two_level_ao_and_dense(ctx, tmp_path, shape):
     ref = O.raymarch(osc, ocam, O.RAY_SHADOW, O.shading(W.LIGHT), threads=8)
     _check(ctx, tg, ref, "dense %s" % (shape,))
     sc.destroy()
+
+
+def test_pin_validates_shapes(ctx):
+    import ctypes as C
+    sc = ctx.scene_generate(L.GEN_SPHERES, L.SCENE_DENSE_U8, 128, 1, 8)
+    cam = R.camera([300.0, 250.0, 200.0], [64.0, 64.0, 64.0], [0.0, 0.0, 1.0], 60.0, 64, 48)
+    t = L.TuneResult()
+    t.size = C.sizeof(t)
+    t.variant, t.tile_w, t.tile_h = 2, 16, 16  # variant 2 has 8x8 tiles
+    assert ctx.lib.meso_autotune_pin_raymarch(ctx.h, sc.h, L.byref(cam), 0, L.byref(t)) == L.ERR_ARG
+    t.variant, t.grid = 5, 33 * ctx.device_info().compute_units
+    assert ctx.lib.meso_autotune_pin_raymarch(ctx.h, sc.h, L.byref(cam), 0, L.byref(t)) == L.ERR_ARG
+    t.variant = 9
+    assert ctx.lib.meso_autotune_pin_raymarch(ctx.h, sc.h, L.byref(cam), 0, L.byref(t)) == L.ERR_ARG
+    # a valid pin renders the same bits as the default shape
+    tg0 = R.RayTargets(ctx, 64, 48)
+    R.raymarch(ctx, sc, cam, 0, None, tg0)
+    t.variant, t.grid, t.tile_w, t.tile_h = 5, 8, 8, 8
+    L.check(ctx.lib.meso_autotune_pin_raymarch(ctx.h, sc.h, L.byref(cam), 0, L.byref(t)))
+    tg1 = R.RayTargets(ctx, 64, 48)
+    R.raymarch(ctx, sc, cam, 0, None, tg1)
+    ctx.sync()
+    for a, b in zip(tg0.results(), tg1.results()):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    sc.destroy()
