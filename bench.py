@@ -178,8 +178,16 @@ def mesh_leg(ctx, R, L, W, steps, warmup, use_graph=True):
 
 def serve_leg(ctx, R, L, W, n_requests, qps, budget_mb):
     from mesoengine_amd import serving
-    return serving.run(ctx, n_requests=n_requests, qps=qps, budget_bytes=budget_mb << 20,
-                       log=log)
+    out = serving.run(ctx, n_requests=n_requests, qps=qps, budget_bytes=budget_mb << 20, log=log)
+    # the same stream from the native load generator: capacity (200 requests
+    # queued at t=0) and latency at the same QPS (1000 requests)
+    cap = serving.run_native(200, 1.0, budget_bytes=budget_mb << 20, log=log, capacity=True)
+    timed = serving.run_native(1000, out["qps"], budget_bytes=budget_mb << 20, log=log)
+    out["native"] = dict(capacity_mrays_served=cap["served"]["mrays"],
+                         capacity_mrays_plain=cap["plain"]["mrays"],
+                         capacity_ratio=round(cap["served"]["mrays"] / cap["plain"]["mrays"], 3),
+                         at_qps=timed)
+    return out
 
 
 def cpu_leg(W, seconds_cap=30.0):

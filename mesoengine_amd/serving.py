@@ -118,6 +118,53 @@ def served_run(ctx, scenes, reqs, sh, budget_bytes, timed_arrivals, rays_per_lau
     return np.array(lat), makespan, stats
 
 
+def write_requests(path, edges, reqs, seed=4, generator=L.GEN_TERRAIN_CAVES,
+                   layout=L.SCENE_BRICKMAP_U16):
+    """The request stream as the native load generator (meso_serve_bench) reads
+    it: scenes (edge, generator, layout, seed) then requests (arrival, scene,
+    size, camera)."""
+    import struct
+    with open(path, "wb") as f:
+        f.write(b"MESOREQS")
+        f.write(struct.pack("<Iii", 1, len(edges), len(reqs)))
+        for i, e in enumerate(edges):
+            f.write(struct.pack("<iiiQ", e, generator, layout, seed + i))
+        for r in reqs:
+            f.write(struct.pack("<diii", r["arrival"], r["scene"], r["width"], r["height"]))
+            f.write(struct.pack("<10f", *r["eye"], *r["target"], *r["up"], r["vfov"]))
+
+
+def run_native(n_requests, qps, budget_bytes=512 << 20, seed=4, log=print, timeout=300,
+               capacity=False):
+    """Replays the seeded request stream with the native load generator (no
+    interpreter in the timed loop): plain and served latency at `qps`, or
+    with every request queued at t=0 (capacity)."""
+    import json
+    import os
+    import subprocess
+    import tempfile
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "meso_serve_bench")
+    if not os.path.exists(exe):
+        raise RuntimeError("meso_serve_bench is not built (make -C mesoengine_amd/engine)")
+    edges = W.serve_scenes(seed)
+    reqs = W.serve_requests(edges, n_requests, qps, seed)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "requests.bin")
+        write_requests(path, edges, reqs, seed)
+        cmd = [exe, path, "--budget-mb", str(budget_bytes >> 20)] + (["--capacity"] if capacity
+                                                                      else [])
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    if p.returncode != 0:
+        raise RuntimeError("meso_serve_bench failed: " + p.stderr[-2000:])
+    out = json.loads(p.stdout.strip().splitlines()[-1])
+    log("native serving (%s, %d requests): served p50 %.2f p99 %.2f ms, %.0f Mrays/s; plain p50 "
+        "%.2f p99 %.2f ms, %.0f Mrays/s" % (
+            "capacity" if capacity else "%.0f QPS" % qps, n_requests, out["served"]["p50_ms"],
+            out["served"]["p99_ms"], out["served"]["mrays"], out["plain"]["p50_ms"],
+            out["plain"]["p99_ms"], out["plain"]["mrays"]))
+    return out
+
+
 def _pct(a, q):
     return float(np.percentile(a, q) * 1e3)
 

@@ -70,3 +70,9 @@ def test_engine_instance_builds_and_rejects_bad_arguments():
     assert p.returncode == 2 and "usage" in p.stderr
     eng = C.CDLL(os.path.join(b, "libmeso_engine.so"))
     assert eng is not None
+    sb = os.path.join(b, "meso_serve_bench")
+    p = subprocess.run([sb], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 2 and "usage" in p.stderr
+    p = subprocess.run([sb, os.path.join(b, "no_such_requests.bin")], capture_output=True,
+                       text=True, timeout=60)
+    assert p.returncode == 1 and "cannot read" in p.stderr

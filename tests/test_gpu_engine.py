@@ -104,3 +104,14 @@ def test_instance_frames_match_oracle(tmp_path, scene, gen, n):
     assert np.array_equal(d["t"].view(np.uint32), ref[2].view(np.uint32))
     assert np.array_equal(d["shade"].view(np.uint32), ref[3].view(np.uint32))
     assert (d["idx"] >= 0).mean() > 0.2, "camera sees too little of the scene"
+
+
+def test_native_serving_driver_runs():
+    """The native load generator replays a short seeded stream on both paths."""
+    from mesoengine_amd import serving
+    out = serving.run_native(40, 400.0, budget_bytes=256 << 20, log=print, timeout=120)
+    assert out["requests"] == 40 and out["scenes"] == 16 and out["rays"] > 0
+    for path in ("plain", "served"):
+        assert 0 < out[path]["p50_ms"] <= out[path]["p99_ms"] <= out[path]["max_ms"] < 5000
+        assert out[path]["mrays"] > 0
+    assert out["served"]["launches"] >= 1
