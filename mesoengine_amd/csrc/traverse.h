@@ -29,6 +29,8 @@ __device__ __forceinline__ int face_axis(int f) { return f < 2 ? 2 : (f < 4 ? 1 
 struct Ray {
   float o[3], d[3], inv[3];
   int32_t step[3];
+  int32_t pos[3];  // 1 where step > 0: the exit plane of block b is b + 1
+  int32_t neg[3];  // 1 where step < 0: the next cell is one below the plane
 };
 
 __device__ __forceinline__ void ray_setup(Ray &r, const float o[3], const float d[3]) {
@@ -46,6 +48,8 @@ __device__ __forceinline__ void ray_setup(Ray &r, const float o[3], const float 
       r.step[a] = 0;
       r.inv[a] = 0.0f;
     }
+    r.pos[a] = r.step[a] > 0 ? 1 : 0;
+    r.neg[a] = r.step[a] < 0 ? 1 : 0;
   }
 }
 
@@ -77,53 +81,52 @@ __device__ __forceinline__ uint32_t lin3u(uint32_t x, uint32_t y, uint32_t z, ui
 }
 
 // One step at level 2^sh (sh = 0 voxel, 3 brick, 5 and 7 empty blocks): leave
-// the block holding the current cell through its nearest plane.  For sh > 0
-// the other two coordinates come from the exit point, clamped into the block
-// and the scene and never moving against the ray; for sh = 0 they stay, which
-// is the plain voxel step.  Returns false when the ray leaves the scene.
+// the block holding the current cell through its nearest plane.  The other
+// two coordinates come from the exit point, clamped into the block and the
+// scene and never moving against the ray; at sh = 0 the block is the cell
+// itself, so the clamp keeps them, which is the plain voxel step.  Written
+// without branches (selects only), so that the lanes of a wave stepping at
+// different levels stay converged.  Returns false when the ray leaves the
+// scene.
+template <bool LEVELS = true>
 __device__ __forceinline__ bool advance(const Ray &r, int32_t n, int32_t sh, WalkState &w) {
   float tb[3];
-  int32_t bc[3];
+  int32_t bc[3], pl[3];
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     bc[a] = w.cell[a] >> sh;
-    if (r.step[a] > 0)
-      tb[a] = plane_t(r.o[a], r.inv[a], (float)((bc[a] + 1) << sh));
-    else if (r.step[a] < 0)
-      tb[a] = plane_t(r.o[a], r.inv[a], (float)(bc[a] << sh));
-    else
-      tb[a] = __builtin_inff();
+    pl[a] = (bc[a] + r.pos[a]) << sh;  // the exit plane on this axis
+    const float t = plane_t(r.o[a], r.inv[a], (float)pl[a]);
+    tb[a] = r.step[a] != 0 ? t : __builtin_inff();
   }
   const int ax = argmin3(tb[0], tb[1], tb[2]);
-  float t = tb[0];
-  int32_t na = 0, st = 0;
-#pragma unroll
-  for (int a = 0; a < 3; a++)
-    if (a == ax) {
-      t = tb[a];
-      st = r.step[a];
-      na = st > 0 ? (bc[a] + 1) << sh : (bc[a] << sh) - 1;
-    }
+  const float t = ax == 0 ? tb[0] : (ax == 1 ? tb[1] : tb[2]);
+  const int32_t st = ax == 0 ? r.step[0] : (ax == 1 ? r.step[1] : r.step[2]);
+  const int32_t na = (ax == 0 ? pl[0] - r.neg[0] : (ax == 1 ? pl[1] - r.neg[1] : pl[2] - r.neg[2]));
   if (na < 0 || na >= n) return false;
-  if (sh > 0) {
+  if constexpr (!LEVELS) {  // voxel steps only (dense grid): the other cells stay
 #pragma unroll
-    for (int b = 0; b < 3; b++) {
-      if (b == ax) continue;
-      float dt = r.d[b] * t;
-      float p = r.o[b] + dt;
-      int32_t c = (int32_t)__builtin_floorf(p);
-      int32_t lo = bc[b] << sh, hi = lo + (1 << sh) - 1;
-      hi = hi > n - 1 ? n - 1 : hi;
-      c = c < lo ? lo : c;
-      c = c > hi ? hi : c;
-      if (r.step[b] > 0 && c < w.cell[b]) c = w.cell[b];
-      if (r.step[b] < 0 && c > w.cell[b]) c = w.cell[b];
-      w.cell[b] = c;
-    }
+    for (int b = 0; b < 3; b++)
+      if (b == ax) w.cell[b] = na;
+    w.t = t;
+    w.face = entry_face(ax, st);
+    return true;
   }
+  const int32_t span = (1 << sh) - 1;
 #pragma unroll
-  for (int a = 0; a < 3; a++)
-    if (a == ax) w.cell[a] = na;
+  for (int b = 0; b < 3; b++) {
+    float dt = r.d[b] * t;
+    float p = r.o[b] + dt;
+    int32_t c = (int32_t)__builtin_floorf(p);
+    const int32_t lo = bc[b] << sh;
+    int32_t hi = lo + span;
+    hi = hi > n - 1 ? n - 1 : hi;
+    c = c < lo ? lo : c;
+    c = c > hi ? hi : c;
+    c = (r.step[b] > 0 && c < w.cell[b]) ? w.cell[b] : c;
+    c = (r.step[b] < 0 && c > w.cell[b]) ? w.cell[b] : c;
+    w.cell[b] = b == ax ? na : c;
+  }
   w.t = t;
   w.face = entry_face(ax, st);
   return true;
@@ -153,7 +156,7 @@ __device__ __forceinline__ bool walk(const DevScene &s, const Ray &r, WalkState 
         material = v;
         return true;
       }
-      if (!advance(r, n, 0, w)) return false;
+      if (!advance<false>(r, n, 0, w)) return false;
     }
   } else {
     const uint32_t n128 = (uint32_t)(n + 127) >> 7, n32 = (uint32_t)(n + 31) >> 5;
