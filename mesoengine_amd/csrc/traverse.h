@@ -29,11 +29,14 @@ __device__ __forceinline__ int face_axis(int f) { return f < 2 ? 2 : (f < 4 ? 1 
 struct Ray {
   float o[3], d[3], inv[3];
   int32_t step[3];
-  int32_t pos[3];  // 1 where step > 0: the exit plane of block b is b + 1
-  int32_t neg[3];  // 1 where step < 0: the next cell is one below the plane
+  int32_t pos[3];    // 1 where step > 0: the exit plane of block b is b + 1
+  uint32_t posmask;  // bit a: step[a] > 0
+  uint32_t negmask;  // bit a: step[a] < 0 (the next cell is one below the plane)
 };
 
 __device__ __forceinline__ void ray_setup(Ray &r, const float o[3], const float d[3]) {
+  r.posmask = 0u;
+  r.negmask = 0u;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     r.o[a] = o[a];
@@ -49,7 +52,8 @@ __device__ __forceinline__ void ray_setup(Ray &r, const float o[3], const float 
       r.inv[a] = 0.0f;
     }
     r.pos[a] = r.step[a] > 0 ? 1 : 0;
-    r.neg[a] = r.step[a] < 0 ? 1 : 0;
+    r.posmask |= (r.step[a] > 0 ? 1u : 0u) << a;
+    r.negmask |= (r.step[a] < 0 ? 1u : 0u) << a;
   }
 }
 
@@ -99,17 +103,20 @@ __device__ __forceinline__ bool advance(const Ray &r, int32_t n, int32_t sh, Wal
     const float t = plane_t(r.o[a], r.inv[a], (float)pl[a]);
     tb[a] = r.step[a] != 0 ? t : __builtin_inff();
   }
+  // the chosen axis by selects and bit extracts only: an index into the
+  // ray's arrays would move them out of registers
   const int ax = argmin3(tb[0], tb[1], tb[2]);
-  const float t = ax == 0 ? tb[0] : (ax == 1 ? tb[1] : tb[2]);
-  const int32_t st = ax == 0 ? r.step[0] : (ax == 1 ? r.step[1] : r.step[2]);
-  const int32_t na = (ax == 0 ? pl[0] - r.neg[0] : (ax == 1 ? pl[1] - r.neg[1] : pl[2] - r.neg[2]));
+  const bool a0 = ax == 0, a1 = ax == 1;
+  const float t = a0 ? tb[0] : (a1 ? tb[1] : tb[2]);
+  const int32_t pax = a0 ? pl[0] : (a1 ? pl[1] : pl[2]);
+  const int32_t na = pax - (int32_t)((r.negmask >> ax) & 1u);
+  const int32_t face = 4 - 2 * ax + (int32_t)((r.posmask >> ax) & 1u);  // entry_face
   if (na < 0 || na >= n) return false;
   if constexpr (!LEVELS) {  // voxel steps only (dense grid): the other cells stay
 #pragma unroll
-    for (int b = 0; b < 3; b++)
-      if (b == ax) w.cell[b] = na;
+    for (int b = 0; b < 3; b++) w.cell[b] = b == ax ? na : w.cell[b];
     w.t = t;
-    w.face = entry_face(ax, st);
+    w.face = face;
     return true;
   }
   const int32_t span = (1 << sh) - 1;
@@ -128,7 +135,7 @@ __device__ __forceinline__ bool advance(const Ray &r, int32_t n, int32_t sh, Wal
     w.cell[b] = b == ax ? na : c;
   }
   w.t = t;
-  w.face = entry_face(ax, st);
+  w.face = face;
   return true;
 }
 
@@ -167,7 +174,10 @@ __device__ __forceinline__ bool walk(const DevScene &s, const Ray &r, WalkState 
     for (;;) {
       if (!(w.t < tmax)) return false;
       if (COUNT) cnt.n_iter++;
-      int32_t sh = 0;
+      // one flat pass over the levels (no nesting, so a wave whose lanes sit
+      // at different levels runs each piece once): refresh a level's cached
+      // answer when the cell left its block, then step at the coarsest empty
+      // level, or stop at a solid voxel
       const uint32_t x = (uint32_t)w.cell[0], y = (uint32_t)w.cell[1], z = (uint32_t)w.cell[2];
       const int32_t k128 = (int32_t)((x >> 7) | ((y >> 7) << 10) | ((z >> 7) << 20));
       if (k128 != c128) {
@@ -184,49 +194,42 @@ __device__ __forceinline__ bool walk(const DevScene &s, const Ray &r, WalkState 
           e128 = !occ_bit(s.occ128, lin3u(x >> 7, y >> 7, z >> 7, n128));
         }
       }
-      if (e128) {
-        sh = 7;
-      } else {
-        const int32_t k32 = (int32_t)((x >> 5) | ((y >> 5) << 10) | ((z >> 5) << 20));
-        if (k32 != c32) {
-          c32 = k32;
-          cb = -1;
-          if constexpr (KIND == 3) {
-            e32 = !((groups >> (((x >> 5) & 3u) + 4u * ((y >> 5) & 3u) + 16u * ((z >> 5) & 3u))) &
-                    1ull);
-          } else {
-            if (COUNT) cnt.n_coarse++;
-            e32 = !occ_bit(s.occ32, lin3u(x >> 5, y >> 5, z >> 5, n32));
-          }
-        }
-        if (e32) {
-          sh = 5;
+      const int32_t k32 = (int32_t)((x >> 5) | ((y >> 5) << 10) | ((z >> 5) << 20));
+      if (!e128 && k32 != c32) {
+        c32 = k32;
+        cb = -1;
+        if constexpr (KIND == 3) {
+          e32 = !((groups >> (((x >> 5) & 3u) + 4u * ((y >> 5) & 3u) + 16u * ((z >> 5) & 3u))) &
+                  1ull);
         } else {
-          const int32_t kb = (int32_t)((x >> 3) | ((y >> 3) << 11) | ((z >> 3) << 22));
-          if (kb != cb) {
-            cb = kb;
-            if constexpr (KIND == 3) {
-              if (COUNT) cnt.n_node++;
-              slot = s.slots[(uint64_t)node * 4096u +
-                             lin3u((x >> 3) & 15u, (y >> 3) & 15u, (z >> 3) & 15u, 16u)];
-            } else {
-              if (COUNT) cnt.n_brickslot++;
-              slot = s.slots[lin3u(x >> 3, y >> 3, z >> 3, (uint32_t)n >> 3)];
-            }
-            if (COUNT && slot != kEmpty) cnt.n_brickmask++;
-          }
-          if (slot == kEmpty) {
-            sh = 3;
-          } else {
-            const uint32_t lx = x & 7u, ly = y & 7u, lz = z & 7u;
-            const uint64_t m = s.masks[(uint64_t)slot * 8u + lz];
-            if ((m >> (lx + 8u * ly)) & 1ull) {
-              material = s.voxels[(uint64_t)slot * 512u + lx + 8u * ly + 64u * lz];
-              return true;
-            }
-          }
+          if (COUNT) cnt.n_coarse++;
+          e32 = !occ_bit(s.occ32, lin3u(x >> 5, y >> 5, z >> 5, n32));
         }
       }
+      const bool coarse = e128 || e32;
+      const int32_t kb = (int32_t)((x >> 3) | ((y >> 3) << 11) | ((z >> 3) << 22));
+      if (!coarse && kb != cb) {
+        cb = kb;
+        if constexpr (KIND == 3) {
+          if (COUNT) cnt.n_node++;
+          slot = s.slots[(uint64_t)node * 4096u +
+                         lin3u((x >> 3) & 15u, (y >> 3) & 15u, (z >> 3) & 15u, 16u)];
+        } else {
+          if (COUNT) cnt.n_brickslot++;
+          slot = s.slots[lin3u(x >> 3, y >> 3, z >> 3, (uint32_t)n >> 3)];
+        }
+        if (COUNT && slot != kEmpty) cnt.n_brickmask++;
+      }
+      const bool in_brick = !coarse && slot != kEmpty;
+      if (in_brick) {
+        const uint32_t lx = x & 7u, ly = y & 7u, lz = z & 7u;
+        const uint64_t m = s.masks[(uint64_t)slot * 8u + lz];
+        if ((m >> (lx + 8u * ly)) & 1ull) {
+          material = s.voxels[(uint64_t)slot * 512u + lx + 8u * ly + 64u * lz];
+          return true;
+        }
+      }
+      const int32_t sh = e128 ? 7 : (e32 ? 5 : (in_brick ? 0 : 3));
       if (!advance(r, n, sh, w)) return false;
     }
   }
