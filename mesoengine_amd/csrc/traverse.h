@@ -1,6 +1,6 @@
 // Device-side voxel traversal for gfx950: Amanatides & Woo (1987) DDA with
 // hierarchical empty-space skipping over 128^3 blocks, 32^3 blocks, 8^3
-// bricks and voxels.  Every float operation is written out in a fixed order
+// bricks, the 4^3 and 2^3 blocks inside a brick, and voxels.  Every float operation is written out in a fixed order
 // and the library is compiled with -ffp-contract=off, so results are the IEEE
 // binary32 results of these operations: the integer outputs (voxel index,
 // face) and the float outputs (t, shade) are reproducible bit for bit by any
@@ -139,6 +139,34 @@ __device__ __forceinline__ bool advance(const Ray &r, int32_t n, int32_t sh, Wal
   return true;
 }
 
+// One bit per aligned 2^3 block of a brick (bit cx + 4 cy + 16 cz), from its
+// 64-byte mask (word z, bit x + 8 y): OR the z pairs, fold x and y pairs
+// onto the even bits, then gather the 16 even bits of each pair plane.
+__device__ __forceinline__ uint64_t brick_blocks2(const uint64_t *mask) {
+  const uint4 *q = (const uint4 *)mask;  // 64-byte aligned
+  uint64_t m[8];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const uint4 v = q[i];
+    m[2 * i] = (uint64_t)v.x | ((uint64_t)v.y << 32);
+    m[2 * i + 1] = (uint64_t)v.z | ((uint64_t)v.w << 32);
+  }
+  uint64_t sub = 0;
+#pragma unroll
+  for (int p = 0; p < 4; p++) {
+    uint64_t w = m[2 * p] | m[2 * p + 1];
+    w |= w >> 1;
+    w |= w >> 8;
+    w &= 0x0055005500550055ull;
+    w = (w | (w >> 1)) & 0x0033003300330033ull;
+    w = (w | (w >> 2)) & 0x000F000F000F000Full;
+    w = (w | (w >> 12)) & 0x000000FF000000FFull;
+    w = (w | (w >> 24)) & 0xFFFFull;
+    sub |= w << (16 * p);
+  }
+  return sub;
+}
+
 __device__ __forceinline__ bool occ_bit(const uint32_t *bits, uint32_t i) {
   return (bits[i >> 5] >> (i & 31u)) & 1u;
 }
@@ -146,7 +174,9 @@ __device__ __forceinline__ bool occ_bit(const uint32_t *bits, uint32_t i) {
 // Walk until a solid voxel is entered (true), the ray leaves the scene or the
 // entry t reaches tmax (false).  KIND: 1 dense u8 (voxel steps only), 2 brick
 // map, 3 two-level.  Each step of a brick scene takes the coarsest empty
-// level holding the cell: 128^3 block, 32^3 block, brick, else the voxel.
+// level holding the cell: 128^3 block, 32^3 block, brick, the 4^3 and 2^3
+// blocks of a non-empty brick (from its mask, read once per brick), else the
+// voxel.
 template <int KIND, bool COUNT>
 __device__ __forceinline__ bool walk(const DevScene &s, const Ray &r, WalkState &w, float tmax,
                                      uint16_t &material, StepCounts &cnt) {
@@ -170,7 +200,7 @@ __device__ __forceinline__ bool walk(const DevScene &s, const Ray &r, WalkState 
     int32_t c128 = -1, c32 = -1, cb = -1;  // packed block keys of the cached levels
     bool e128 = false, e32 = false;
     uint32_t node = kEmpty, slot = kEmpty;
-    uint64_t groups = 0;
+    uint64_t groups = 0, sub = 0;  // sub: 2^3-block bits of the current brick
     for (;;) {
       if (!(w.t < tmax)) return false;
       if (COUNT) cnt.n_iter++;
@@ -218,18 +248,28 @@ __device__ __forceinline__ bool walk(const DevScene &s, const Ray &r, WalkState 
           if (COUNT) cnt.n_brickslot++;
           slot = s.slots[lin3u(x >> 3, y >> 3, z >> 3, (uint32_t)n >> 3)];
         }
-        if (COUNT && slot != kEmpty) cnt.n_brickmask++;
-      }
-      const bool in_brick = !coarse && slot != kEmpty;
-      if (in_brick) {
-        const uint32_t lx = x & 7u, ly = y & 7u, lz = z & 7u;
-        const uint64_t m = s.masks[(uint64_t)slot * 8u + lz];
-        if ((m >> (lx + 8u * ly)) & 1ull) {
-          material = s.voxels[(uint64_t)slot * 512u + lx + 8u * ly + 64u * lz];
-          return true;
+        if (slot != kEmpty) {
+          if (COUNT) cnt.n_brickmask++;
+          sub = brick_blocks2(s.masks + (uint64_t)slot * 8u);
         }
       }
-      const int32_t sh = e128 ? 7 : (e32 ? 5 : (in_brick ? 0 : 3));
+      // inside a non-empty brick: step over its empty 4^3 and 2^3 blocks
+      const bool in_brick = !coarse && slot != kEmpty;
+      bool e4 = false, e2 = false;
+      if (in_brick) {
+        const uint32_t lx = x & 7u, ly = y & 7u, lz = z & 7u;
+        e4 = ((sub >> (2u * (lx >> 2) + 8u * (ly >> 2) + 32u * (lz >> 2))) & 0x00330033ull) == 0;
+        e2 = ((sub >> ((lx >> 1) + 4u * (ly >> 1) + 16u * (lz >> 1))) & 1ull) == 0;
+        if (!e4 && !e2) {
+          const uint64_t m = s.masks[(uint64_t)slot * 8u + lz];
+          if ((m >> (lx + 8u * ly)) & 1ull) {
+            material = s.voxels[(uint64_t)slot * 512u + lx + 8u * ly + 64u * lz];
+            return true;
+          }
+        }
+      }
+      const int32_t sh =
+          e128 ? 7 : (e32 ? 5 : (!in_brick ? 3 : (e4 ? 2 : (e2 ? 1 : 0))));
       if (!advance(r, n, sh, w)) return false;
     }
   }

@@ -563,10 +563,22 @@ static int same3(const int32_t a[3], const int32_t b[3]) {
   return a[0] == b[0] && a[1] == b[1] && a[2] == b[2];
 }
 
+/* 1 if the aligned B^3 block (B = 2 or 4) of the brick that holds voxel
+ * (lx, ly, lz) has no solid voxel (mask word z, bit x + 8 y). */
+static int brick_block_empty(const uint64_t *mask, int32_t lx, int32_t ly, int32_t lz, int32_t B) {
+  const int32_t x0 = lx & ~(B - 1), y0 = ly & ~(B - 1), z0 = lz & ~(B - 1);
+  for (int32_t z = z0; z < z0 + B; z++)
+    for (int32_t y = y0; y < y0 + B; y++)
+      for (int32_t x = x0; x < x0 + B; x++)
+        if ((mask[z] >> (x + 8 * y)) & 1u) return 0;
+  return 1;
+}
+
 /* Walk from state w until a solid voxel is entered (returns 1), the ray
  * leaves the scene or the entry t reaches tmax (returns 0).  Each step picks
  * the coarsest empty level holding the current cell: 128^3 block, 32^3
- * block, brick, else the voxel itself. */
+ * block, brick, 4^3 and 2^3 blocks inside a non-empty brick (read from its
+ * mask), else the voxel itself. */
 static int walk(const mo_scene *s, const ray_t *r, walk_state *w, float tmax, uint16_t *material,
                 mo_counters *cnt) {
   const int32_t n = s->n;
@@ -634,7 +646,11 @@ static int walk(const mo_scene *s, const ray_t *r, walk_state *w, float tmax, ui
           } else {
             int32_t lx = w->cell[0] & 7, ly = w->cell[1] & 7, lz = w->cell[2] & 7;
             const uint64_t *mask = s->masks + (int64_t)slot * 8;
-            if ((mask[lz] >> (lx + 8 * ly)) & 1u) {
+            if (brick_block_empty(mask, lx, ly, lz, 4)) {
+              L = 4;
+            } else if (brick_block_empty(mask, lx, ly, lz, 2)) {
+              L = 2;
+            } else if ((mask[lz] >> (lx + 8 * ly)) & 1u) {
               *material = s->voxels[(int64_t)slot * 512 + lx + 8 * ly + 64 * lz];
               return 1;
             }

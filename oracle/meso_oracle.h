@@ -173,7 +173,12 @@ typedef struct {
   uint16_t material;
 } mo_hit;
 
-/* First solid voxel along o + t d inside [0,n)^3.  Origin anywhere. */
+/* First solid voxel along o + t d inside [0,n)^3.  Origin anywhere.  The
+ * walk steps at the coarsest empty level that holds the current cell: 128^3
+ * and 32^3 blocks, bricks, and inside a non-empty brick its aligned 4^3 and
+ * 2^3 blocks, else the voxel.  A step at level L leaves the block through its
+ * nearest plane; the other coordinates come from the exit point, clamped into
+ * the block and never moving against the ray. */
 int mo_trace(const mo_scene *s, const float o[3], const float d[3], mo_hit *hit,
              mo_counters *cnt);
 /* Occlusion ray from a point on a face of voxel `cell`, starting in the empty
