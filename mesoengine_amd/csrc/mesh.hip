@@ -155,6 +155,22 @@ struct MeshLds {
   uint32_t base_lo, base_hi;
 };
 
+// 32x32 bit transpose across the 32 lanes of a half wave: lane y holds row y
+// (bit x = element [y][x]) and gets back column y (bit x = element [x][y]).
+// Stage j swaps the high-j bits of lane y (y & j == 0) with the low-j bits of
+// lane y + j.
+__device__ __forceinline__ uint32_t transpose32(uint32_t w, uint32_t y) {
+  constexpr uint32_t hi[5] = {0xFFFF0000u, 0xFF00FF00u, 0xF0F0F0F0u, 0xCCCCCCCCu, 0xAAAAAAAAu};
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const int j = 16 >> k;
+    const uint32_t p = (uint32_t)__shfl_xor((int)w, j, 32);
+    const uint32_t h = hi[k];
+    w = (y & (uint32_t)j) ? ((w & h) | ((p >> j) & ~h)) : ((w & ~h) | ((p << j) & h));
+  }
+  return w;
+}
+
 // OR of the 8-bit pieces held by the 4 lanes of a row (lanes 4r .. 4r+3)
 __device__ __forceinline__ uint32_t row_or(uint32_t piece) {
   piece |= __shfl_xor(piece, 1, 64);
@@ -200,26 +216,18 @@ __device__ __forceinline__ void build_masks(MeshLds &L, const Rows &a) {
     }
   }
   __syncthreads();
-  // ballot transpose: lane = y + 32 * (z & 1) holds row [z][y]; the ballot of
-  // bit x over the lanes is column x (bits over y) of planes z and z+1.
+  // transposes: lane y + 32 * (z & 1) holds row [z][y] (bits over x) of two z
+  // planes; five block-swap stages between lanes y and y ^ j (j = 16 .. 1)
+  // turn it into column [z][x] (bits over y), one per lane
   for (uint32_t zp = wave; zp < 16u; zp += nwaves) {
     const uint32_t z = zp * 2u + (lane >> 5);
     const uint32_t y = lane & 31u;
-    const uint32_t o = L.M[kO + z * kZY + y], ey = L.M[kEy + z * kZY + y],
-                   ez = L.M[kEz + z * kZY + y];
-    for (uint32_t xb = 0; xb < 32u; xb++) {
-      const uint64_t t0 = __ballot((o >> xb) & 1u);
-      const uint64_t t1 = __ballot((ey >> xb) & 1u);
-      const uint64_t t2 = __ballot((ez >> xb) & 1u);
-      if (lane == 0) {
-        const uint32_t z0 = zp * 2u;
-        L.M[kOT + z0 * 32u + xb] = (uint32_t)t0;
-        L.M[kOT + (z0 + 1u) * 32u + xb] = (uint32_t)(t0 >> 32);
-        L.M[kEyT + z0 * 32u + xb] = (uint32_t)t1;
-        L.M[kEyT + (z0 + 1u) * 32u + xb] = (uint32_t)(t1 >> 32);
-        L.M[kEzT + z0 * 32u + xb] = (uint32_t)t2;
-        L.M[kEzT + (z0 + 1u) * 32u + xb] = (uint32_t)(t2 >> 32);
-      }
+    const uint32_t rows[3] = {L.M[kO + z * kZY + y], L.M[kEy + z * kZY + y],
+                              L.M[kEz + z * kZY + y]};
+    constexpr int dst[3] = {kOT, kEyT, kEzT};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      L.M[dst[k] + z * 32u + y] = transpose32(rows[k], y);
     }
   }
   __syncthreads();
@@ -328,15 +336,22 @@ __device__ __forceinline__ void close_quads(const MeshLds &L, int t, int f, int 
                                             uint32_t open, uint32_t starts, int vend, uint32_t *out,
                                             uint64_t base, uint64_t cap) {
   const uint32_t cont = open & ~starts;
+  // two quads per trip: both records are read before either is used
   while (bits) {
-    const int u = __ffs(bits) - 1;
+    const int u1 = __ffs(bits) - 1;
     bits &= bits - 1;
-    const uint32_t above = u < 31 ? ~(cont >> (u + 1)) : 0u;  // zero where the segment ends
-    const int w = 1 + (u < 31 ? (__ffs(above) - 1) : 0);
-    const uint32_t r = L.rec[u][t];
-    const int v0 = (int)(r & 31u);
-    const uint64_t k = base + (r >> 5);
-    if (k < cap) out[k] = pack_quad(f, s, u, v0, w, vend - v0);
+    const bool two = bits != 0u;
+    const int u2 = two ? __ffs(bits) - 1 : u1;
+    bits &= two ? bits - 1 : bits;
+    const uint32_t r1 = L.rec[u1][t], r2 = L.rec[u2][t];
+    const uint32_t a1 = u1 < 31 ? ~(cont >> (u1 + 1)) : 0u;  // zero where the segment ends
+    const uint32_t a2 = u2 < 31 ? ~(cont >> (u2 + 1)) : 0u;
+    const int w1 = 1 + (u1 < 31 ? (__ffs(a1) - 1) : 0);
+    const int w2 = 1 + (u2 < 31 ? (__ffs(a2) - 1) : 0);
+    const int v1 = (int)(r1 & 31u), v2 = (int)(r2 & 31u);
+    const uint64_t k1 = base + (r1 >> 5), k2 = base + (r2 >> 5);
+    if (k1 < cap) out[k1] = pack_quad(f, s, u1, v1, w1, vend - v1);
+    if (two && k2 < cap) out[k2] = pack_quad(f, s, u2, v2, w2, vend - v2);
   }
 }
 
@@ -350,11 +365,17 @@ __device__ __forceinline__ void plane_emit(MeshLds &L, int t, uint32_t *out, uin
   if (!GREEDY) {
     for (int v = 0; v < 32; v++) {
       uint32_t row = face_row(L, p, v);
+      // two faces per trip
       while (row) {
-        const int u = __ffs(row) - 1;
+        const int u1 = __ffs(row) - 1;
         row &= row - 1;
-        const uint64_t k = base + count++;
-        if (k < cap) out[k] = pack_quad(f, s, u, v, 1, 1);
+        const bool two = row != 0u;
+        const int u2 = two ? __ffs(row) - 1 : u1;
+        row &= two ? row - 1 : row;
+        const uint64_t k = base + count;
+        count += two ? 2u : 1u;
+        if (k < cap) out[k] = pack_quad(f, s, u1, v, 1, 1);
+        if (two && k + 1 < cap) out[k + 1] = pack_quad(f, s, u2, v, 1, 1);
       }
     }
     return;
@@ -374,9 +395,14 @@ __device__ __forceinline__ void plane_emit(MeshLds &L, int t, uint32_t *out, uin
     open |= avail;
     starts |= st;
     while (st) {
-      const int u = __ffs(st) - 1;
+      const int u1 = __ffs(st) - 1;
       st &= st - 1;
-      L.rec[u][t] = (uint32_t)v | (count++ << 5);
+      const bool two = st != 0u;
+      const int u2 = two ? __ffs(st) - 1 : u1;
+      st &= two ? st - 1 : st;
+      L.rec[u1][t] = (uint32_t)v | (count << 5);
+      if (two) L.rec[u2][t] = (uint32_t)v | ((count + 1u) << 5);
+      count += two ? 2u : 1u;
     }
     ev_prev = L.M[p.ev + v * p.stride];
   }
