@@ -77,6 +77,8 @@ def raymarch_leg(ctx, R, L, W, steps, warmup):
     e1.sync()
     wall = time.perf_counter() - wall0
     ms = e0.elapsed_ms(e1) / steps
+    # the same frames with their outputs copied to pinned host memory each frame
+    pcie = pcie_inclusive(ctx, L, R, sc, cam, flags, sh, tg, w * h, max(3, steps // 2))
     rays = cnt["rays"] + cnt["shadow_rays"]
     # bytes per frame: 4 B per brick slot visited, 64 B per non-empty brick
     # mask fetched, 2 B material per hit, 13 B of outputs per pixel
@@ -87,9 +89,34 @@ def raymarch_leg(ctx, R, L, W, steps, warmup):
                shadow=cnt["shadow_rays"], counts=cnt, alg_bytes=alg_bytes, achieved_gbs=achieved,
                mrays=rays / (ms * 1e-3) / 1e6, mrays_primary=cnt["rays"] / (ms * 1e-3) / 1e6,
                tune=dict(variant=tune.variant, grid=tune.grid, tile=[tune.tile_w, tune.tile_h],
-                         best_ms=round(tune.best_ms, 4), default_ms=round(tune.default_ms, 4)))
+                         best_ms=round(tune.best_ms, 4), default_ms=round(tune.default_ms, 4)),
+               pcie_inclusive=pcie)
     sc.destroy()
     return res
+
+
+def pcie_inclusive(ctx, L, R, sc, cam, flags, sh, tg, px, frames):
+    """Frame time when every frame's outputs (index, face, t, shade) are also
+    copied to pinned host memory before the next frame starts."""
+    import ctypes as C
+    sizes = [(tg.index, 4 * px), (tg.face, px), (tg.t, 4 * px), (tg.shade, 4 * px)]
+    host = []
+    for _, nb in sizes:
+        p = C.c_void_p()
+        L.check(ctx.lib.meso_host_alloc(ctx.h, nb, C.byref(p)))
+        host.append(p)
+    try:
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            R.raymarch(ctx, sc, cam, flags, sh, tg)
+            for (buf, nb), hp in zip(sizes, host):
+                L.check(ctx.lib.meso_memcpy_d2h(ctx.h, hp, buf.ptr, nb, None))
+        ms = (time.perf_counter() - t0) * 1e3 / frames
+    finally:
+        for hp in host:
+            ctx.lib.meso_host_free(ctx.h, hp)
+    nbytes = sum(nb for _, nb in sizes)
+    return dict(ms_per_frame=round(ms, 4), bytes_to_host=nbytes, frames=frames)
 
 
 def mesh_leg(ctx, R, L, W, steps, warmup, use_graph=True):
@@ -346,6 +373,8 @@ def main():
         "cpu_baseline": cpu,
         "mrays_primary": round(rm["mrays_primary"], 2),
         "raymarch_tune": rm["tune"],
+        "raymarch_pcie_inclusive": dict(rm["pcie_inclusive"], mrays=round(
+            rm["rays"] / (rm["pcie_inclusive"]["ms_per_frame"] * 1e-3) / 1e6, 1)),
         "meshing": mesh,
         "serving": serve,
         "frame": frame,

@@ -116,6 +116,11 @@ meso_status meso_memcpy_d2h(meso_context *ctx, void *dst, const void *src, uint6
                             meso_stream stream);
 meso_status meso_memcpy_h2d(meso_context *ctx, void *dst, const void *src, uint64_t bytes,
                             meso_stream stream);
+/* Pinned (page-locked) host memory for full-speed copies between host and
+ * device (the host-visible storage behind IContext::getMappedPtr,
+ * LVK.h:823).  Free with meso_host_free. */
+meso_status meso_host_alloc(meso_context *ctx, uint64_t bytes, void **out);
+meso_status meso_host_free(meso_context *ctx, void *ptr);
 /* device-to-device copy, asynchronous on the stream.  lvk has no buffer to
  * buffer copy (only IContext::upload, LVK.h:822); this one moves results
  * between device buffers and is the known-size copy that calibrates the

@@ -116,6 +116,8 @@ _SIGS = {
     "meso_memcpy_d2h": (C.c_int32, [P, P, P, C.c_uint64, P]),
     "meso_memcpy_h2d": (C.c_int32, [P, P, P, C.c_uint64, P]),
     "meso_memcpy_d2d": (C.c_int32, [P, P, P, C.c_uint64, P]),
+    "meso_host_alloc": (C.c_int32, [P, C.c_uint64, P]),
+    "meso_host_free": (C.c_int32, [P, P]),
     "meso_scene_create_dense_u8": (C.c_int32, [P, C.c_int32, P, PP]),
     "meso_scene_create_brickmap_u16": (C.c_int32, [P, C.c_int32, P, C.c_int64, P, P, PP]),
     "meso_scene_create_two_level_u16": (C.c_int32, [P, C.c_int32, P, C.c_int64, P, C.c_int64,

@@ -414,6 +414,20 @@ meso_status meso_memcpy_h2d(meso_context *ctx, void *dst, const void *src, uint6
   return MESO_OK;
 }
 
+meso_status meso_host_alloc(meso_context *ctx, uint64_t bytes, void **out) {
+  MESO_CHECK(ctx && out && bytes, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument or size 0");
+  *out = nullptr;
+  MESO_HIP(hipSetDevice(ctx->device));
+  MESO_HIP(hipHostMalloc(out, bytes, hipHostMallocDefault));
+  return MESO_OK;
+}
+
+meso_status meso_host_free(meso_context *ctx, void *ptr) {
+  MESO_CHECK(ctx, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  if (ptr) MESO_HIP(hipHostFree(ptr));
+  return MESO_OK;
+}
+
 meso_status meso_memcpy_d2d(meso_context *ctx, void *dst, const void *src, uint64_t bytes,
                             meso_stream stream) {
   MESO_CHECK(ctx && dst && src, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");

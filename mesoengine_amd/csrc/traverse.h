@@ -446,8 +446,70 @@ struct PixelResult {
   float shade;
 };
 
-// Primary ray of pixel (px,py), optional shadow ray toward the light and AO
-// rays, shaded as (0.25 + 0.75 * lambert * visible) * (0.5 + 0.5 * unoccluded).
+// Secondary rays of a primary hit (voxel `cell`, entry face `face` < 6, at t
+// along o + t d): the shadow ray toward the light when the face is lit, the
+// AO rays, and the shade (0.25 + 0.75 * lambert * visible) * (0.5 + 0.5 *
+// unoccluded fraction).  With neither flag set it traces nothing.
+template <int KIND, bool COUNT>
+__device__ __forceinline__ float shade_hit(const DevScene &s, uint32_t flags, const DevShading &sh,
+                                           uint32_t pix, const float o[3], const float d[3],
+                                           int32_t face, float t, const int32_t cell[3],
+                                           StepCounts &cnt, uint32_t &n_shadow, uint32_t &n_ao) {
+  const int na = face_axis(face);
+  const int sg = (face & 1) ? -1 : 1;
+  float lightc = sh.light[0];
+  int32_t start[3] = {cell[0], cell[1], cell[2]};
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    if (k == na) {
+      lightc = sh.light[k];
+      start[k] += sg;
+    }
+  float lambert = sg > 0 ? lightc : -lightc;
+  if (lambert < 0.0f) lambert = 0.0f;
+  float p[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    float dt = d[i] * t;
+    p[i] = o[i] + dt;
+  }
+  float vis = 1.0f;
+  if ((flags & MESO_RAY_SHADOW) && lambert > 0.0f) {
+    n_shadow++;
+    if (occluded<KIND, COUNT>(s, p, sh.light, start, __builtin_inff(), cnt)) vis = 0.0f;
+  }
+  float ao = 1.0f;
+  if (flags & MESO_RAY_AO) {
+    int32_t unocc = 0;
+    for (int32_t k = 0; k < sh.ao_samples; k++) {
+      float dir[3];
+      ao_direction(sh.ao_seed, pix, (uint32_t)k, face, dir);
+      n_ao++;
+      if (!occluded<KIND, COUNT>(s, p, dir, start, sh.ao_radius, cnt)) unocc++;
+    }
+    float frac = (float)unocc / (float)sh.ao_samples;
+    frac = 0.5f * frac;
+    ao = 0.5f + frac;
+  }
+  float l = lambert * vis;
+  l = 0.75f * l;
+  float shade = 0.25f + l;
+  shade = shade * ao;
+  return shade;
+}
+
+// true when a primary hit on `face` needs secondary rays
+__device__ __forceinline__ bool needs_secondary(uint32_t flags, const DevShading &sh, int32_t face) {
+  if (face >= 6) return false;
+  if (flags & MESO_RAY_AO) return true;
+  if (!(flags & MESO_RAY_SHADOW)) return false;
+  const int na = face_axis(face);
+  const float lightc = na == 0 ? sh.light[0] : (na == 1 ? sh.light[1] : sh.light[2]);
+  const float lambert = (face & 1) ? -lightc : lightc;
+  return lambert > 0.0f;
+}
+
+// Primary ray of pixel (px,py), then its secondary rays and shade.
 template <int KIND, bool COUNT>
 __device__ __forceinline__ PixelResult shade_pixel(const DevScene &s, const DevCamera &cam,
                                                    uint32_t flags, const DevShading &sh,
@@ -460,51 +522,11 @@ __device__ __forceinline__ PixelResult shade_pixel(const DevScene &s, const DevC
   const bool hit = trace<KIND, COUNT>(s, o, d, h, cnt);
   float shade = 0.0f;
   if (hit) {
-    if (h.face < 6) {
-      const int na = face_axis(h.face);
-      const int sg = (h.face & 1) ? -1 : 1;
-      float lightc = sh.light[0];
-      int32_t start[3] = {h.cell[0], h.cell[1], h.cell[2]};
-#pragma unroll
-      for (int k = 0; k < 3; k++)
-        if (k == na) {
-          lightc = sh.light[k];
-          start[k] += sg;
-        }
-      float lambert = sg > 0 ? lightc : -lightc;
-      if (lambert < 0.0f) lambert = 0.0f;
-      float p[3];
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        float dt = d[i] * h.t;
-        p[i] = o[i] + dt;
-      }
-      float vis = 1.0f;
-      if ((flags & MESO_RAY_SHADOW) && lambert > 0.0f) {
-        n_shadow++;
-        if (occluded<KIND, COUNT>(s, p, sh.light, start, __builtin_inff(), cnt)) vis = 0.0f;
-      }
-      float ao = 1.0f;
-      if (flags & MESO_RAY_AO) {
-        int32_t unocc = 0;
-        const uint32_t pix = (uint32_t)((int64_t)py * cam.width + px);
-        for (int32_t k = 0; k < sh.ao_samples; k++) {
-          float dir[3];
-          ao_direction(sh.ao_seed, pix, (uint32_t)k, h.face, dir);
-          n_ao++;
-          if (!occluded<KIND, COUNT>(s, p, dir, start, sh.ao_radius, cnt)) unocc++;
-        }
-        float frac = (float)unocc / (float)sh.ao_samples;
-        frac = 0.5f * frac;
-        ao = 0.5f + frac;
-      }
-      float l = lambert * vis;
-      l = 0.75f * l;
-      shade = 0.25f + l;
-      shade = shade * ao;
-    } else {
+    if (h.face < 6)
+      shade = shade_hit<KIND, COUNT>(s, flags, sh, (uint32_t)((int64_t)py * cam.width + px), o, d,
+                                     h.face, h.t, h.cell, cnt, n_shadow, n_ao);
+    else
       shade = 0.25f;
-    }
   }
   res.index = h.index;
   res.face = h.face;

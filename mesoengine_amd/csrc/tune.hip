@@ -197,8 +197,8 @@ meso_status meso_autotune_raymarch(meso_context *ctx, meso_scene *scene, const m
   const int cus = ctx->props.multiProcessorCount;
   struct Shape {
     int v, tw, th, grid;
-  } shapes[] = {{0, 16, 16, 0}, {1, 32, 8, 0},         {2, 8, 8, 0},         {3, 16, 8, 0},
-                {4, 32, 16, 0}, {5, 8, 8, 0},          {5, 8, 8, 24 * cus},  {5, 8, 8, 16 * cus}};
+  } shapes[] = {{0, 16, 16, 0}, {1, 32, 8, 0}, {2, 8, 8, 0},         {3, 16, 8, 0},
+                {4, 32, 16, 0}, {5, 8, 8, 0},  {5, 8, 8, 24 * cus}, {5, 8, 8, 16 * cus}};
   meso_tune_result best{};
   best.size = sizeof(best);
   best.best_ms = 1e30f;

@@ -5,9 +5,10 @@ Two steps:
   run    the workload profiled under `rocprofv3 --pmc FETCH_SIZE` and again
          under `--pmc WRITE_SIZE`: three device-to-device copies of a known
          size (the calibration), three 1920x1080 primary + shadow frames of
-         the 512^3 heightfield brick map, two batched greedy and two batched
-         culled meshings of the 4096-chunk batch, and one 3840x2160 frame
-         (primary + shadow + 4 AO) of the 2048^3 two-level caves;
+         the 512^3 heightfield brick map in the autotuned launch shape (after
+         the tuner's own candidate launches), two batched greedy and two
+         batched culled meshings of the 4096-chunk batch, and one 3840x2160
+         frame (primary + shadow + 4 AO) of the 2048^3 two-level caves;
   parse  reads both counter CSVs and writes a JSON summary: per kernel the
          counter's KiB per dispatch, scaled by the factor that maps the
          copy's reading to the bytes it moved (read = written = copy size).
@@ -34,11 +35,24 @@ COPY_BYTES = 1 << 30
 # kernel-name prefix -> key in the summary
 KERNELS = {
     "__amd_rocclr_copyBuffer": "copy_calibration",
-    "void meso::raymarch_kernel<2,": "raymarch_brickmap_terrain512",
-    "void meso::raymarch_kernel<3,": "raymarch_two_level_caves2048",
+    "void meso::raymarch_persistent_kernel<2, false>": "raymarch_brickmap_terrain512",
+    "void meso::raymarch_kernel<2, false,": "raymarch_brickmap_terrain512_tiled",
+    "void meso::raymarch_persistent_kernel<3, false>": "raymarch_two_level_caves2048",
+    "void meso::raymarch_kernel<3, false,": "raymarch_two_level_caves2048_tiled",
     "void meso::mesh_batched_kernel<2, true, 0>": "mesh_batched_greedy_4096",
     "void meso::mesh_batched_kernel<2, false, 0>": "mesh_batched_culled_4096",
 }
+
+
+def _tune(ctx, L, sc, cam, flags, sh, tg):
+    """the launch shape bench.py times: the autotuner's pick (its candidate
+    launches are profiled too and land in the per-kernel averages)"""
+    import ctypes as C
+    t = L.TuneResult()
+    t.size = C.sizeof(t)
+    L.check(ctx.lib.meso_autotune_raymarch(ctx.h, sc.h, L.byref(cam), flags, L.byref(sh),
+                                           L.byref(tg.out), L.byref(t)))
+    print("traffic: tuned variant %d grid %d" % (t.variant, t.grid), flush=True)
 
 
 def run():
@@ -63,6 +77,7 @@ def run():
     cp = W.terrain_camera()
     cam = R.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], cfg["width"], cfg["height"])
     tg = R.RayTargets(ctx, cfg["width"], cfg["height"])
+    _tune(ctx, L, sc, cam, L.RAY_SHADOW, R.shading(W.LIGHT), tg)
     for _ in range(3):
         R.raymarch(ctx, sc, cam, L.RAY_SHADOW, R.shading(W.LIGHT), tg)
     ctx.sync()
@@ -87,8 +102,9 @@ def run():
     cp = W.caves_camera()
     cam = R.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], cfg["width"], cfg["height"])
     tg = R.RayTargets(ctx, cfg["width"], cfg["height"], index_bytes=8)
-    R.raymarch(ctx, sc, cam, L.RAY_SHADOW | L.RAY_AO,
-               R.shading(W.LIGHT, 5, cfg["ao_samples"], cfg["ao_radius"]), tg)
+    sh = R.shading(W.LIGHT, 5, cfg["ao_samples"], cfg["ao_radius"])
+    _tune(ctx, L, sc, cam, L.RAY_SHADOW | L.RAY_AO, sh, tg)
+    R.raymarch(ctx, sc, cam, L.RAY_SHADOW | L.RAY_AO, sh, tg)
     ctx.sync()
     sc.destroy()
     ctx.close()

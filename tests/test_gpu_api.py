@@ -225,3 +225,21 @@ def test_argument_errors(ctx):
     assert lib.meso_buffer_upload(buf.h, m.ctypes.data, 1 << 30, 0) == L.ERR_ARG
     assert lib.meso_buffer_device_address(buf.h, 3) == 0
     sc.destroy()
+
+
+def test_pinned_host_copies(ctx):
+    data = np.arange(1 << 20, dtype=np.uint32)
+    buf = ctx.upload(data)
+    p = C.c_void_p()
+    L.check(ctx.lib.meso_host_alloc(ctx.h, data.nbytes, C.byref(p)))
+    try:
+        L.check(ctx.lib.meso_memcpy_d2h(ctx.h, p, buf.ptr, data.nbytes, None))
+        got = np.ctypeslib.as_array((C.c_uint32 * data.size).from_address(p.value))
+        assert np.array_equal(got, data)
+        dst = ctx.buffer(data.nbytes)
+        L.check(ctx.lib.meso_memcpy_d2d(ctx.h, dst.ptr, buf.ptr, data.nbytes, None))
+        ctx.sync()
+        assert np.array_equal(dst.download(np.uint32), data)
+    finally:
+        L.check(ctx.lib.meso_host_free(ctx.h, p))
+    assert ctx.lib.meso_host_alloc(ctx.h, 0, C.byref(p)) == L.ERR_ARG
