@@ -31,6 +31,9 @@
  *   DDA: brute-force FP64 slab intersection against every solid voxel on
  *     tiny grids; closed forms (single voxel, axis-aligned box, axis-parallel
  *     rays); FP64 entry-plane t of the chosen voxel.
+ *   Shading: FP64 recomputation of every hit pixel's shade (lambert, shadow
+ *     and AO occlusion by brute-force slab tests against every solid voxel).
+ *   Edits / scene chunks: numpy restatement over the dense grid.
  *   Meshers: brute-force face enumeration; closed forms (a x b x c box);
  *     greedy tiling invariants; erosion-cull cross-check.
  *   Reference TestGenerator (sin hash): parity unpinned (CPU timing only).
