@@ -72,10 +72,11 @@ struct RayOut {
 // raymarch launch shape (autotuner variants)
 struct RayLaunch {
   int32_t variant;  // 0..4: 8x8 wave packets in 16x16, 32x8, 8x8, 16x8, 32x16 tiles;
-                    // 5: persistent waves pulling 8x8 packets from per-XCD band queues
+                    // 5, 6, 7: persistent waves pulling 8x8, 16x4, 32x2 packets from
+                    // per-XCD band queues
   int32_t block;
   int32_t tile_w, tile_h;
-  int32_t grid;     // variant 5: workgroups (0 = one per wave slot of the device)
+  int32_t grid;     // variants 5-7: workgroups (0 = one per wave slot of the device)
 };
 
 struct MeshLaunch {

@@ -65,16 +65,18 @@ __global__ void __launch_bounds__(TW *TH) raymarch_kernel(DevScene s, DevCamera 
 }
 
 // Persistent waves: one 64-lane workgroup per wave slot of the device.  The
-// screen's 8x8 packets are cut into 8 bands of rows; the waves of XCD x
+// screen's PW x (64 / PW) packets (8x8, 16x4 or 32x2) are cut into 8 bands
+// of rows; the waves of XCD x
 // (workgroup i runs on XCD i % 8) take packets of band x from that band's
 // counter, then help the other bands in turn.  A wave leaves once every
 // counter has passed the end of its band, so the grid always drains; slots
 // freed by short (sky) packets are refilled at once instead of waiting for
 // the next workgroup dispatch.
-template <int KIND, bool COUNT>
+template <int KIND, bool COUNT, int PW>
 __global__ void __launch_bounds__(64) raymarch_persistent_kernel(
     DevScene s, DevCamera cam, uint32_t flags, DevShading sh, RayOut out, uint32_t packets_x,
     uint32_t n_packets, unsigned int *queue) {
+  constexpr uint32_t PH = 64 / PW;  // packet of PW x PH pixels
   const uint32_t lane = threadIdx.x;
   const uint32_t home = blockIdx.x & 7u;
   StepCounts cnt = {0, 0, 0, 0, 0, 0, 0};
@@ -88,8 +90,8 @@ __global__ void __launch_bounds__(64) raymarch_persistent_kernel(
       if (lane == 0) p = atomicAdd(&queue[band * 16u], 1u);  // one counter per 64 B line
       p = lo + (uint32_t)__shfl((int)p, 0, 64);
       if (p >= hi) break;  // uniform over the wave
-      const int32_t px = (int32_t)((p % packets_x) * 8u + (lane & 7u));
-      const int32_t py = (int32_t)((p / packets_x) * 8u + (lane >> 3));
+      const int32_t px = (int32_t)((p % packets_x) * (uint32_t)PW + lane % (uint32_t)PW);
+      const int32_t py = (int32_t)((p / packets_x) * PH + lane / (uint32_t)PW);
       if (px < cam.width && py < cam.height) {
         const PixelResult r = shade_pixel<KIND, COUNT>(s, cam, flags, sh, px, py, cnt, n_shadow, n_ao);
         const int64_t pix = (int64_t)py * cam.width + px;
@@ -119,16 +121,17 @@ __global__ void __launch_bounds__(64) raymarch_persistent_kernel(
   }
 }
 
-template <int KIND, bool COUNT>
+template <int KIND, bool COUNT, int PW>
 static meso_status launch_persistent(meso_context *ctx, const meso_scene *scene,
                                      const DevCamera &cam, uint32_t flags, const DevShading &sh,
                                      const RayOut &out, hipStream_t stream, int grid) {
-  const uint32_t packets_x = (uint32_t)((cam.width + 7) / 8);
-  const uint32_t n_packets = packets_x * (uint32_t)((cam.height + 7) / 8);
+  constexpr int PH = 64 / PW;
+  const uint32_t packets_x = (uint32_t)((cam.width + PW - 1) / PW);
+  const uint32_t n_packets = packets_x * (uint32_t)((cam.height + PH - 1) / PH);
   if (grid <= 0) {
     int per_cu = 0;
     MESO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, raymarch_persistent_kernel<KIND, COUNT>, 64, 0));
+        &per_cu, raymarch_persistent_kernel<KIND, COUNT, PW>, 64, 0));
     grid = (per_cu < 1 ? 1 : per_cu) * ctx->props.multiProcessorCount;
   }
   grid = (grid + 7) & ~7;  // every XCD gets the same number of waves
@@ -138,7 +141,7 @@ static meso_status launch_persistent(meso_context *ctx, const meso_scene *scene,
   MESO_HIP(hipMallocAsync((void **)&queue, 8 * 64, stream));
   meso_status st = launch_zero(queue, 8 * 64, stream);
   if (st == MESO_OK) {
-    hipLaunchKernelGGL((raymarch_persistent_kernel<KIND, COUNT>), dim3((uint32_t)grid), dim3(64), 0,
+    hipLaunchKernelGGL((raymarch_persistent_kernel<KIND, COUNT, PW>), dim3((uint32_t)grid), dim3(64), 0,
                        stream, scene->dev(), cam, flags, sh, out, packets_x, n_packets, queue);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) st = fail(MESO_ERR_RUNTIME, "raymarch launch: %s", hipGetErrorString(e));
@@ -171,7 +174,9 @@ static meso_status launch_kind(meso_context *ctx, const meso_scene *scene, const
     case 2: return launch_shape<KIND, COUNT, 8, 8>(scene, cam, flags, sh, out, stream);
     case 3: return launch_shape<KIND, COUNT, 16, 8>(scene, cam, flags, sh, out, stream);
     case 4: return launch_shape<KIND, COUNT, 32, 16>(scene, cam, flags, sh, out, stream);
-    case 5: return launch_persistent<KIND, COUNT>(ctx, scene, cam, flags, sh, out, stream, grid);
+    case 5: return launch_persistent<KIND, COUNT, 8>(ctx, scene, cam, flags, sh, out, stream, grid);
+    case 6: return launch_persistent<KIND, COUNT, 16>(ctx, scene, cam, flags, sh, out, stream, grid);
+    case 7: return launch_persistent<KIND, COUNT, 32>(ctx, scene, cam, flags, sh, out, stream, grid);
     default: return fail(MESO_ERR_ARGUMENT_OUT_OF_RANGE, "unknown raymarch variant %d", variant);
   }
 }

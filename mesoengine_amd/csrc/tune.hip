@@ -192,13 +192,15 @@ meso_status meso_autotune_raymarch(meso_context *ctx, meso_scene *scene, const m
       (shading->light[0] != 0.0f || shading->light[1] != 0.0f || shading->light[2] != 0.0f))
     derive_shading(shading, &sh);
   RayOut ro{outp->hit_index, outp->index_bytes, outp->face, outp->t, outp->shade, nullptr};
-  // tiled shapes, then persistent waves at full occupancy (grid 0) and at
-  // 24 / 16 waves per CU
+  // tiled shapes, then persistent waves with 8x8, 16x4 and 32x2 packets at
+  // full occupancy (grid 0) and at 24 (16) waves per CU
   const int cus = ctx->props.multiProcessorCount;
   struct Shape {
     int v, tw, th, grid;
-  } shapes[] = {{0, 16, 16, 0}, {1, 32, 8, 0}, {2, 8, 8, 0},         {3, 16, 8, 0},
-                {4, 32, 16, 0}, {5, 8, 8, 0},  {5, 8, 8, 24 * cus}, {5, 8, 8, 16 * cus}};
+  } shapes[] = {{0, 16, 16, 0},      {1, 32, 8, 0},       {2, 8, 8, 0},
+                {3, 16, 8, 0},       {4, 32, 16, 0},      {5, 8, 8, 0},
+                {5, 8, 8, 24 * cus}, {5, 8, 8, 16 * cus}, {6, 16, 4, 0},
+                {6, 16, 4, 24 * cus}, {7, 32, 2, 0},      {7, 32, 2, 24 * cus}};
   meso_tune_result best{};
   best.size = sizeof(best);
   best.best_ms = 1e30f;
@@ -214,7 +216,7 @@ meso_status meso_autotune_raymarch(meso_context *ctx, meso_scene *scene, const m
     if (ms < best.best_ms) {
       best.best_ms = ms;
       best.variant = sp.v;
-      best.block = sp.v == 5 ? 64 : sp.tw * sp.th;
+      best.block = sp.v >= 5 ? 64 : sp.tw * sp.th;
       best.grid = sp.grid;
       best.tile_w = sp.tw;
       best.tile_h = sp.th;
@@ -237,22 +239,22 @@ meso_status meso_autotune_pin_raymarch(meso_context *ctx, meso_scene *scene, con
   MESO_CHECK(ctx && scene && cam && shape, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
   MESO_CHECK(cam->size >= sizeof(meso_camera) && shape->size >= sizeof(meso_tune_result),
              MESO_ERR_ARGUMENT_OUT_OF_RANGE, "struct size too small");
-  static const int tiles[5][2] = {{16, 16}, {32, 8}, {8, 8}, {16, 8}, {32, 16}};
+  static const int tiles[8][2] = {{16, 16}, {32, 8}, {8, 8}, {16, 8}, {32, 16},
+                                  {8, 8},   {16, 4}, {32, 2}};
   meso_tune_result r = *shape;
-  if (r.variant >= 0 && r.variant <= 4) {
-    MESO_CHECK(r.tile_w == tiles[r.variant][0] && r.tile_h == tiles[r.variant][1],
-               MESO_ERR_ARGUMENT_OUT_OF_RANGE, "variant %d has %dx%d tiles", r.variant,
-               tiles[r.variant][0], tiles[r.variant][1]);
+  MESO_CHECK(r.variant >= 0 && r.variant <= 7, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "unknown raymarch variant %d", r.variant);
+  MESO_CHECK(r.tile_w == tiles[r.variant][0] && r.tile_h == tiles[r.variant][1],
+             MESO_ERR_ARGUMENT_OUT_OF_RANGE, "variant %d has %dx%d %s", r.variant,
+             tiles[r.variant][0], tiles[r.variant][1], r.variant <= 4 ? "tiles" : "packets");
+  if (r.variant <= 4) {
     r.block = r.tile_w * r.tile_h;
     r.grid = 0;
   } else {
-    MESO_CHECK(r.variant == 5, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "unknown raymarch variant %d",
-               r.variant);
     MESO_CHECK(r.grid >= 0 && r.grid <= 32 * ctx->props.multiProcessorCount,
                MESO_ERR_ARGUMENT_OUT_OF_RANGE, "persistent grid %d outside 0..%d", r.grid,
                32 * ctx->props.multiProcessorCount);
     r.block = 64;
-    r.tile_w = r.tile_h = 8;
   }
   char key[128];
   snprintf(key, sizeof(key), "ray:%d:%d:%dx%d:%u", scene->kind, scene->n, cam->width, cam->height,
@@ -291,7 +293,7 @@ meso_status meso_autotune_load(meso_context *ctx, const char *path) {
     if (!(is >> key >> r.variant >> r.block >> r.grid >> r.tile_w >> r.tile_h >> r.best_ms >>
           r.default_ms))
       continue;
-    if (r.variant < 0 || r.variant > 5 || r.grid < 0 ||
+    if (r.variant < 0 || r.variant > 7 || r.grid < 0 ||
         r.grid > 32 * ctx->props.multiProcessorCount)
       continue;
     ctx->tune[key] = r;

@@ -1,9 +1,10 @@
 """Every raymarch launch shape gives the oracle's bits and step counts.
 
-The five tiled shapes and the persistent waves (variant 5, at the device's
-full occupancy and at a small grid that makes every wave serve several
-bands) are forced through the autotuner's cache file, then run through
-meso_raymarch with the step counters on.
+The five tiled shapes and the persistent waves on 8x8, 16x4 and 32x2
+packets (variants 5-7, at the device's full occupancy and at a small grid
+that makes every wave serve several bands) are forced through the
+autotuner's cache file, then run through meso_raymarch with the step
+counters on.
 """
 import numpy as np
 import pytest
@@ -17,14 +18,14 @@ pytestmark = pytest.mark.gpu
 
 # (variant, tile_w, tile_h, grid)
 SHAPES = [(0, 16, 16, 0), (1, 32, 8, 0), (2, 8, 8, 0), (3, 16, 8, 0), (4, 32, 16, 0),
-          (5, 8, 8, 0), (5, 8, 8, 64)]
+          (5, 8, 8, 0), (5, 8, 8, 64), (6, 16, 4, 0), (7, 32, 2, 64)]
 
 
 def _force(ctx, tmp_path, kind, n, w, h, flags, shape):
     v, tw, th, grid = shape
     key = "ray:%d:%d:%dx%d:%d" % (kind, n, w, h, flags)
     p = tmp_path / "tune.txt"
-    p.write_text("%s %d %d %d %d %d 1 1\n" % (key, v, 64 if v == 5 else tw * th, grid, tw, th))
+    p.write_text("%s %d %d %d %d %d 1 1\n" % (key, v, 64 if v >= 5 else tw * th, grid, tw, th))
     L.check(ctx.lib.meso_autotune_load(ctx.h, str(p).encode()))
 
 
@@ -58,7 +59,7 @@ def test_brickmap_shapes_match_oracle(ctx, tmp_path, shape):
     sc.destroy()
 
 
-@pytest.mark.parametrize("shape", [(5, 8, 8, 0), (5, 8, 8, 24)])
+@pytest.mark.parametrize("shape", [(5, 8, 8, 0), (5, 8, 8, 24), (6, 16, 4, 24), (7, 32, 2, 0)])
 def test_persistent_two_level_ao_and_dense(ctx, tmp_path, shape):
     n, w, h = 256, 136, 72
     sc = ctx.scene_generate(L.GEN_CAVES, L.SCENE_TWO_LEVEL_U16, n, 5)
