@@ -58,6 +58,25 @@ meso_status launch_zero(void *ptr, uint64_t bytes, hipStream_t stream) {
   return MESO_OK;
 }
 
+meso_status band_queue(meso_context *ctx, hipStream_t stream, unsigned int **out) {
+  std::lock_guard<std::mutex> lk(ctx->queue_mu);
+  auto it = ctx->band_queues.find(stream);
+  if (it != ctx->band_queues.end()) {
+    *out = it->second;
+    return MESO_OK;
+  }
+  unsigned int *q = nullptr;
+  MESO_HIP(hipMalloc((void **)&q, 1024));
+  const hipError_t e = hipMemset(q, 0, 1024);
+  if (e != hipSuccess) {
+    hipFree(q);
+    return fail(MESO_ERR_RUNTIME, "hipMemset: %s", hipGetErrorString(e));
+  }
+  ctx->band_queues[stream] = q;
+  *out = q;
+  return MESO_OK;
+}
+
 // ------------------------------------------------------------------ empty-space levels
 // Brick map: one bit per 32^3 and per 128^3 voxel block holding a brick.
 __global__ void coarse_brickmap_kernel(const uint32_t *slots, uint32_t nb, uint32_t *occ32,
@@ -263,9 +282,14 @@ meso_status meso_context_destroy(meso_context *ctx) {
     hipStreamSynchronize(ctx->stream);
     hipStreamDestroy(ctx->stream);
   }
+  if (!ctx->band_queues.empty()) {
+    hipDeviceSynchronize();
+    for (auto &kv : ctx->band_queues) hipFree(kv.second);
+  }
   delete ctx;
   return MESO_OK;
 }
+
 
 meso_status meso_device_info_get(meso_context *ctx, meso_device_info *out) {
   MESO_CHECK(ctx && out, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");

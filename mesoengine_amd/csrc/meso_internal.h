@@ -93,6 +93,10 @@ struct meso_context {
   hipDeviceProp_t props{};
   std::mutex tune_mu;
   std::map<std::string, meso_tune_result> tune;
+  // per stream: the band counters of the persistent raymarch (zero between
+  // launches: the last wave of each launch resets them)
+  std::mutex queue_mu;
+  std::map<hipStream_t, unsigned int *> band_queues;
 };
 
 struct meso_buffer {
@@ -150,6 +154,8 @@ void derive_camera(const meso_camera *c, DevCamera *out);
 void derive_shading(const meso_shading *s, DevShading *out);
 hipStream_t stream_of(meso_context *ctx, meso_stream s);
 meso_status launch_zero(void *ptr, uint64_t bytes, hipStream_t stream);
+// the persistent raymarch's band counters for `stream` (1 KiB, zeroed once)
+meso_status band_queue(meso_context *ctx, hipStream_t stream, unsigned int **out);
 // builds the empty-space levels (occ32/occ128 or node groups) of a brick scene
 meso_status build_coarse(meso_scene *sc, hipStream_t stream);
 void free_scene_arrays(meso_scene *sc);

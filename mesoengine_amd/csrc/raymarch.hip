@@ -71,7 +71,9 @@ __global__ void __launch_bounds__(TW *TH) raymarch_kernel(DevScene s, DevCamera 
 // counter, then help the other bands in turn.  A wave leaves once every
 // counter has passed the end of its band, so the grid always drains; slots
 // freed by short (sky) packets are refilled at once instead of waiting for
-// the next workgroup dispatch.
+// the next workgroup dispatch.  The counters persist per stream: the last
+// wave to leave (queue[128] counts leavers) sets them back to zero, after
+// every wave's last counter access has returned, for the next launch.
 template <int KIND, bool COUNT, int PW>
 __global__ void __launch_bounds__(64) raymarch_persistent_kernel(
     DevScene s, DevCamera cam, uint32_t flags, DevShading sh, RayOut out, uint32_t packets_x,
@@ -119,6 +121,12 @@ __global__ void __launch_bounds__(64) raymarch_persistent_kernel(
       if (lane == 0 && out.counters && x) atomicAdd(&out.counters[c], x);
     }
   }
+  if (lane == 0 && atomicAdd(&queue[128], 1u) == gridDim.x - 1u) {
+#pragma unroll
+    for (uint32_t b = 0; b < 8u; b++)
+      __hip_atomic_store(&queue[b * 16u], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&queue[128], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 template <int KIND, bool COUNT, int PW>
@@ -135,21 +143,15 @@ static meso_status launch_persistent(meso_context *ctx, const meso_scene *scene,
     grid = (per_cu < 1 ? 1 : per_cu) * ctx->props.multiProcessorCount;
   }
   grid = (grid + 7) & ~7;  // every XCD gets the same number of waves
-  // the band counters live for this launch only (stream-ordered), so
-  // launches on different streams never share them
+  // one counter set per stream: launches on one stream are ordered, and
+  // launches on different streams never share counters
   unsigned int *queue = nullptr;
-  MESO_HIP(hipMallocAsync((void **)&queue, 8 * 64, stream));
-  meso_status st = launch_zero(queue, 8 * 64, stream);
-  if (st == MESO_OK) {
-    hipLaunchKernelGGL((raymarch_persistent_kernel<KIND, COUNT, PW>), dim3((uint32_t)grid), dim3(64), 0,
-                       stream, scene->dev(), cam, flags, sh, out, packets_x, n_packets, queue);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) st = fail(MESO_ERR_RUNTIME, "raymarch launch: %s", hipGetErrorString(e));
-  }
-  const hipError_t ef = hipFreeAsync(queue, stream);
-  if (st == MESO_OK && ef != hipSuccess)
-    st = fail(MESO_ERR_RUNTIME, "hipFreeAsync: %s", hipGetErrorString(ef));
-  return st;
+  const meso_status st = band_queue(ctx, stream, &queue);
+  if (st != MESO_OK) return st;
+  hipLaunchKernelGGL((raymarch_persistent_kernel<KIND, COUNT, PW>), dim3((uint32_t)grid), dim3(64), 0,
+                     stream, scene->dev(), cam, flags, sh, out, packets_x, n_packets, queue);
+  MESO_HIP(hipGetLastError());
+  return MESO_OK;
 }
 
 template <int KIND, bool COUNT, int TW, int TH>
