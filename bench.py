@@ -119,6 +119,24 @@ def pcie_inclusive(ctx, L, R, sc, cam, flags, sh, tg, px, frames):
     return dict(ms_per_frame=round(ms, 4), bytes_to_host=nbytes, frames=frames)
 
 
+def upload_leg(n=512, per=16384, batches=8):
+    """Host-edited bricks uploaded through the engine's pinned double-buffered
+    ring (meso_instance --upload-bricks): bricks/s and GB/s over PCIe, with
+    the scene checked afterwards."""
+    import subprocess
+    exe = os.path.join(ROOT, "mesoengine_amd", "_build", "meso_instance")
+    cmd = [exe, "--scene", "caves", "--n", str(n), "--frames", "1", "--dirty", "8", "--width", "64",
+           "--height", "64", "--upload-bricks", str(per), "--upload-batches", str(batches)]
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    if p.returncode != 0:
+        raise RuntimeError("meso_instance upload failed: " + p.stderr[-2000:])
+    up = json.loads(p.stdout.strip().splitlines()[-1])["upload"]
+    up["scene"] = "%d^3 two-level caves" % n
+    log("brick upload: %d x %d bricks in %.1f ms, %.2f Mbricks/s, %.1f GB/s, check %s" % (
+        batches, per, up["ms"], up["mbricks_s"], up["gb_s"], up["check"]))
+    return up
+
+
 def mesh_leg(ctx, R, L, W, steps, warmup, use_graph=True):
     n = W.CHUNKS["n_chunks"]
     vox = ctx.buffer(2 * 32768 * n)
@@ -343,6 +361,7 @@ def main():
         frame = F.run(ctx, frames=args.frames, warmup=2, serve_qps=qps,
                       serve_requests=args.serve_requests, budget_bytes=args.budget_mb << 20,
                       log=log)
+    upload = None if args.no_frame else upload_leg()
     cpu = None if args.no_cpu else cpu_leg(W)
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic.json")
@@ -378,6 +397,7 @@ def main():
         "meshing": mesh,
         "serving": serve,
         "frame": frame,
+        "brick_upload": upload,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

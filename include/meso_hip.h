@@ -116,6 +116,11 @@ meso_status meso_memcpy_d2h(meso_context *ctx, void *dst, const void *src, uint6
                             meso_stream stream);
 meso_status meso_memcpy_h2d(meso_context *ctx, void *dst, const void *src, uint64_t bytes,
                             meso_stream stream);
+/* Host-to-device copy queued on the stream without waiting: src must stay
+ * valid and unchanged until the stream reaches the copy (pinned memory from
+ * meso_host_alloc copies at full PCIe speed). */
+meso_status meso_memcpy_h2d_async(meso_context *ctx, void *dst, const void *src, uint64_t bytes,
+                                  meso_stream stream);
 /* Pinned (page-locked) host memory for full-speed copies between host and
  * device (the host-visible storage behind IContext::getMappedPtr,
  * LVK.h:823).  Free with meso_host_free. */
@@ -188,6 +193,14 @@ meso_status meso_scene_download(meso_scene *scene, int32_t part, void *dst, uint
  * keeps a zero mask.  d_voxels: n_edit * 512 u16. */
 meso_status meso_scene_update_bricks(meso_scene *scene, const uint32_t *d_slots,
                                      const uint16_t *d_voxels, int32_t n_edit, meso_stream stream);
+/* The same without waiting, for a pipelined upload of host edits (the role
+ * of the double-buffered modify queues and staging upload of
+ * FChunkPool, Runtimes/Voxel/Chunk/ChunkPool.h:137-168,641-705): slots >=
+ * the scene's brick count are skipped and counted into *d_skipped (device
+ * u32, may be NULL).  The slots of one call must be distinct. */
+meso_status meso_scene_update_bricks_async(meso_scene *scene, const uint32_t *d_slots,
+                                           const uint16_t *d_voxels, int32_t n_edit,
+                                           uint32_t *d_skipped, meso_stream stream);
 /* Seeded edits of 32^3 chunks (chunk coordinates d_chunk_xyz[3k], device):
  * inside a sphere drawn per chunk from lowbias32(seed ^ lowbias32(chunk
  * index)) every voxel of an allocated brick toggles (solid -> 0, empty -> 7)

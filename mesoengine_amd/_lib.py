@@ -116,6 +116,7 @@ _SIGS = {
     "meso_memcpy_d2h": (C.c_int32, [P, P, P, C.c_uint64, P]),
     "meso_memcpy_h2d": (C.c_int32, [P, P, P, C.c_uint64, P]),
     "meso_memcpy_d2d": (C.c_int32, [P, P, P, C.c_uint64, P]),
+    "meso_memcpy_h2d_async": (C.c_int32, [P, P, P, C.c_uint64, P]),
     "meso_host_alloc": (C.c_int32, [P, C.c_uint64, P]),
     "meso_host_free": (C.c_int32, [P, P]),
     "meso_scene_create_dense_u8": (C.c_int32, [P, C.c_int32, P, PP]),
@@ -126,6 +127,7 @@ _SIGS = {
     "meso_scene_get_info": (C.c_int32, [P, P]),
     "meso_scene_download": (C.c_int32, [P, C.c_int32, P, C.c_uint64]),
     "meso_scene_update_bricks": (C.c_int32, [P, P, P, C.c_int32, P]),
+    "meso_scene_update_bricks_async": (C.c_int32, [P, P, P, C.c_int32, P, P]),
     "meso_scene_edit_chunks": (C.c_int32, [P, P, C.c_int32, C.c_uint32, P]),
     "meso_mesh_scene_chunks": (C.c_int32, [P, P, P, C.c_int32, C.c_uint32, P, P, P, C.c_uint64,
                                            P, P, P]),

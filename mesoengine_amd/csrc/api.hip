@@ -452,6 +452,14 @@ meso_status meso_host_free(meso_context *ctx, void *ptr) {
   return MESO_OK;
 }
 
+meso_status meso_memcpy_h2d_async(meso_context *ctx, void *dst, const void *src, uint64_t bytes,
+                                  meso_stream stream) {
+  MESO_CHECK(ctx && dst && src, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  hipStream_t s = stream_of(ctx, stream);
+  MESO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+  return MESO_OK;
+}
+
 meso_status meso_memcpy_d2d(meso_context *ctx, void *dst, const void *src, uint64_t bytes,
                             meso_stream stream) {
   MESO_CHECK(ctx && dst && src, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");

@@ -350,7 +350,7 @@ __global__ void __launch_bounds__(512) brick_update_kernel(const uint32_t *edit_
   if (e >= n_edit) return;
   const uint32_t slot = edit_slots[e];
   if ((int64_t)slot >= n_bricks) {  // every lane takes the same branch
-    if (threadIdx.x == 0) atomicAdd(bad, 1u);
+    if (threadIdx.x == 0 && bad) atomicAdd(bad, 1u);
     return;
   }
   const uint16_t v = edit_vox[e * 512u + threadIdx.x];
@@ -674,6 +674,23 @@ meso_status meso_scene_update_bricks(meso_scene *scene, const uint32_t *d_slots,
   MESO_HIP(hipFreeAsync(bad, s));
   MESO_HIP(hipStreamSynchronize(s));
   MESO_CHECK(nbad == 0, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "%u edits named a slot >= n_bricks", nbad);
+  return MESO_OK;
+}
+
+meso_status meso_scene_update_bricks_async(meso_scene *scene, const uint32_t *d_slots,
+                                           const uint16_t *d_voxels, int32_t n_edit,
+                                           uint32_t *d_skipped, meso_stream stream) {
+  MESO_CHECK(scene && d_slots && d_voxels, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  MESO_CHECK(scene->kind != MESO_SCENE_DENSE_U8, MESO_ERR_NOT_SUPPORTED,
+             "brick edits need a brick map or two-level scene");
+  MESO_CHECK(n_edit >= 0, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "negative edit count");
+  if (n_edit == 0) return MESO_OK;
+  hipStream_t s = stream_of(scene->ctx, stream);
+  MESO_HIP(hipSetDevice(scene->ctx->device));
+  hipLaunchKernelGGL(brick_update_kernel, grid_1d((uint64_t)n_edit), dim3(512), 0, s, d_slots,
+                     d_voxels, scene->n_bricks, scene->masks, scene->voxels, d_skipped,
+                     (uint64_t)n_edit);
+  MESO_HIP(hipGetLastError());
   return MESO_OK;
 }
 

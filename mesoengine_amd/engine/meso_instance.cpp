@@ -249,4 +249,94 @@ meso_status FHipVoxelInstance::DownloadMesh(std::vector<uint32_t> &counts,
   return st;
 }
 
+// ------------------------------------------------------------------ uploads
+FBrickUploadRing::FBrickUploadRing(meso_context *ctx, meso_scene *scene, int32_t capacity_bricks,
+                                   int32_t depth)
+    : ctx_(ctx), scene_(scene), capacity_(capacity_bricks), depth_(depth < 2 ? 2 : depth) {}
+
+FBrickUploadRing::~FBrickUploadRing() {
+  Drain();
+  for (Batch &b : batches_) {
+    if (b.h_slots) meso_host_free(ctx_, b.h_slots);
+    if (b.h_voxels) meso_host_free(ctx_, b.h_voxels);
+    if (b.d_slots) meso_buffer_destroy(b.d_slots);
+    if (b.d_voxels) meso_buffer_destroy(b.d_voxels);
+    if (b.done) meso_event_destroy(b.done);
+  }
+  if (d_skipped_) meso_buffer_destroy(d_skipped_);
+}
+
+meso_status FBrickUploadRing::Init() {
+  if (!ctx_ || !scene_ || capacity_ <= 0) return MESO_ERR_ARGUMENT_OUT_OF_RANGE;
+  batches_.resize((size_t)depth_);
+  const uint64_t n = (uint64_t)capacity_;
+  for (Batch &b : batches_) {
+    meso_status st = meso_host_alloc(ctx_, n * 4, (void **)&b.h_slots);
+    if (st == MESO_OK) st = meso_host_alloc(ctx_, n * 1024, (void **)&b.h_voxels);
+    if (st == MESO_OK) st = meso_buffer_create(ctx_, n * 4, nullptr, &b.d_slots);
+    if (st == MESO_OK) st = meso_buffer_create(ctx_, n * 1024, nullptr, &b.d_voxels);
+    if (st == MESO_OK) st = meso_event_create(ctx_, &b.done);
+    if (st != MESO_OK) return st;
+  }
+  meso_status st = meso_buffer_create(ctx_, 16, nullptr, &d_skipped_);
+  if (st == MESO_OK) st = meso_buffer_fill_zero(d_skipped_, nullptr);
+  return st;
+}
+
+meso_status FBrickUploadRing::Begin() {
+  cur_ = (cur_ + 1) % depth_;
+  Batch &b = batches_[(size_t)cur_];
+  if (b.in_flight) {  // the GPU may still read this batch's host memory
+    const meso_status st = meso_event_sync(b.done);
+    if (st != MESO_OK) return st;
+    b.in_flight = false;
+  }
+  b.count = 0;
+  return MESO_OK;
+}
+
+bool FBrickUploadRing::Add(uint32_t slot, const uint16_t *voxels) {
+  if (cur_ < 0) return false;
+  Batch &b = batches_[(size_t)cur_];
+  if (b.count >= capacity_) return false;
+  b.h_slots[b.count] = slot;
+  std::memcpy(b.h_voxels + (size_t)b.count * 512u, voxels, 1024);
+  b.count++;
+  return true;
+}
+
+meso_status FBrickUploadRing::Submit(meso_stream stream) {
+  if (cur_ < 0) return MESO_ERR_ARGUMENT_OUT_OF_RANGE;
+  Batch &b = batches_[(size_t)cur_];
+  if (b.count == 0) return MESO_OK;
+  void *ds = (void *)meso_buffer_device_address(b.d_slots, 0);
+  void *dv = (void *)meso_buffer_device_address(b.d_voxels, 0);
+  meso_status st = meso_memcpy_h2d_async(ctx_, ds, b.h_slots, (uint64_t)b.count * 4, stream);
+  if (st == MESO_OK)
+    st = meso_memcpy_h2d_async(ctx_, dv, b.h_voxels, (uint64_t)b.count * 1024, stream);
+  if (st == MESO_OK)
+    st = meso_scene_update_bricks_async(
+        scene_, (const uint32_t *)ds, (const uint16_t *)dv, b.count,
+        (uint32_t *)meso_buffer_device_address(d_skipped_, 0), stream);
+  if (st == MESO_OK) st = meso_event_record(b.done, stream);
+  if (st == MESO_OK) b.in_flight = true;
+  return st;
+}
+
+meso_status FBrickUploadRing::Drain() {
+  for (Batch &b : batches_)
+    if (b.in_flight) {
+      const meso_status st = meso_event_sync(b.done);
+      if (st != MESO_OK) return st;
+      b.in_flight = false;
+    }
+  return MESO_OK;
+}
+
+uint64_t FBrickUploadRing::Skipped() {
+  if (Drain() != MESO_OK || !d_skipped_) return 0;
+  uint32_t v = 0;
+  return meso_buffer_download(d_skipped_, &v, 4, 0) == MESO_OK ? v : 0;
+}
+
 }  // namespace meso_engine

@@ -73,6 +73,48 @@ class FVoxelCamera {
   Callback CameraChunkUpdateCallback, CameraUpdateCallback;
 };
 
+// Double-buffered upload of bricks edited on the host: the engine fills one
+// pinned batch while the GPU copies and applies the others (the role of the
+// reference's double-buffered modify queues and staging upload,
+// Runtimes/Voxel/Chunk/ChunkPool.h:137-168,641-705, here as pinned host
+// batches, async copies and an async brick update on one stream).
+class FBrickUploadRing {
+ public:
+  FBrickUploadRing(meso_context *ctx, meso_scene *scene, int32_t capacity_bricks, int32_t depth = 2);
+  ~FBrickUploadRing();
+  FBrickUploadRing(const FBrickUploadRing &) = delete;
+  FBrickUploadRing &operator=(const FBrickUploadRing &) = delete;
+
+  meso_status Init();
+  // makes the next batch writable, waiting until the GPU is done with it
+  meso_status Begin();
+  // queues one brick (slot id, 512 u16 voxels x + 8y + 64z); false when full
+  bool Add(uint32_t slot, const uint16_t *voxels);
+  // queues the batch's copies and the brick update on `stream`; no wait
+  meso_status Submit(meso_stream stream = nullptr);
+  // waits for every submitted batch
+  meso_status Drain();
+  int32_t Capacity() const { return capacity_; }
+  // bricks skipped (slot out of range) over all finished batches
+  uint64_t Skipped();
+
+ private:
+  struct Batch {
+    uint32_t *h_slots = nullptr;
+    uint16_t *h_voxels = nullptr;
+    meso_buffer *d_slots = nullptr, *d_voxels = nullptr;
+    meso_event *done = nullptr;
+    int32_t count = 0;
+    bool in_flight = false;
+  };
+  meso_context *ctx_;
+  meso_scene *scene_;
+  int32_t capacity_, depth_;
+  std::vector<Batch> batches_;
+  meso_buffer *d_skipped_ = nullptr;
+  int32_t cur_ = -1;
+};
+
 struct FFrameStats {
   float physics_ms = 0.0f;  // edits + remesh
   float render_ms = 0.0f;

@@ -4,14 +4,17 @@
 // RenderStart, Render, RenderEnd, UpdateFrameIndex).  The camera flies a
 // fixed step along its forward vector every frame, and a seeded set of dirty
 // chunks is edited and remeshed in UpdatePhysics.  One JSON line reports the
-// per-frame timings; --dump writes the last frame's outputs, the last remesh
-// and every frame's dirty set to a little-endian binary file:
+// per-frame timings (and, with --upload-bricks, the double-buffered upload of
+// host-edited bricks, checked against the scene afterwards); --dump writes the
+// last frame's outputs, the last remesh and every frame's dirty set to a
+// little-endian binary file:
 //   "MESOINST" u32 version=1
 //   i32 width, height, index_bytes, frames
 //   f32 eye[3], target[3], up[3], vfov      (the last frame's ray camera)
 //   per frame: u32 edit_seed, i32 n_dirty, i32 xyz[3 n_dirty]
 //   u32 n_chunks, u32 counts[n_chunks], u64 total, u32 quads[total]
 //   index[width*height*index_bytes], u8 face[], f32 t[], f32 shade[]
+#include <chrono>
 #include <cinttypes>
 #include <cmath>
 #include <cstdio>
@@ -52,6 +55,102 @@ std::vector<int32_t> dirty_set(int32_t nc, int32_t count, uint64_t seed, uint32_
   return xyz;
 }
 
+// Uploads `batches` batches of `per` host-edited bricks through the
+// double-buffered ring, then checks the scene's voxels and masks of every
+// touched brick against the last payload written to it.
+bool run_upload(FHipVoxelInstance &inst, int32_t per, int32_t batches, uint64_t seed,
+                std::string &out) {
+  meso_scene_info info{};
+  info.size = sizeof(info);
+  if (meso_scene_get_info(inst.Scene(), &info) != MESO_OK || info.n_bricks <= 0) {
+    out = "no bricks";
+    return false;
+  }
+  const uint64_t nb = (uint64_t)info.n_bricks;
+  if ((uint64_t)per > nb) per = (int32_t)nb;
+  FBrickUploadRing ring(inst.Context(), inst.Scene(), per, 2);
+  if (ring.Init() != MESO_OK) {
+    out = meso_last_error_message();
+    return false;
+  }
+  auto payload = [&](uint32_t slot, uint32_t batch, uint16_t *v) {
+    uint64_t st = seed * 0x9E3779B97F4A7C15ull ^ ((uint64_t)batch << 40) ^ slot;
+    for (int i = 0; i < 512; i += 4) {
+      const uint64_t r = splitmix64(st);
+      for (int k = 0; k < 4; k++) {
+        const uint16_t x = (uint16_t)(r >> (16 * k));
+        v[i + k] = (x & 1u) ? (uint16_t)(1u + (x >> 1) % 7u) : (uint16_t)0;
+      }
+    }
+  };
+  // the edits are made before the timed loop: it measures the upload path
+  // (copy into the pinned batch, async copies, brick update), not the editor
+  std::vector<int32_t> last(nb, -1);  // batch that last wrote each slot
+  std::vector<uint16_t> vox(512);
+  std::vector<uint32_t> slots((size_t)per * batches);
+  std::vector<uint16_t> edits((size_t)per * batches * 512);
+  uint64_t s = seed + 77u;
+  for (int32_t b = 0; b < batches; b++) {
+    std::unordered_set<uint32_t> used;
+    size_t k = (size_t)b * per;
+    while ((int32_t)used.size() < per) {
+      const uint32_t slot = (uint32_t)(splitmix64(s) % nb);
+      if (!used.insert(slot).second) continue;
+      slots[k] = slot;
+      payload(slot, (uint32_t)b, &edits[k * 512]);
+      last[slot] = b;
+      k++;
+    }
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  for (int32_t b = 0; b < batches; b++) {
+    if (ring.Begin() != MESO_OK) {
+      out = meso_last_error_message();
+      return false;
+    }
+    for (size_t k = (size_t)b * per; k < (size_t)(b + 1) * per; k++) ring.Add(slots[k], &edits[k * 512]);
+    if (ring.Submit() != MESO_OK) {
+      out = meso_last_error_message();
+      return false;
+    }
+  }
+  if (ring.Drain() != MESO_OK) {
+    out = meso_last_error_message();
+    return false;
+  }
+  const double ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  // check every touched brick
+  std::vector<uint16_t> all(nb * 512);
+  std::vector<uint64_t> masks(nb * 8);
+  if (meso_scene_download(inst.Scene(), MESO_PART_VOXELS, all.data(), all.size() * 2) != MESO_OK ||
+      meso_scene_download(inst.Scene(), MESO_PART_MASKS, masks.data(), masks.size() * 8) != MESO_OK) {
+    out = meso_last_error_message();
+    return false;
+  }
+  bool ok = ring.Skipped() == 0;
+  uint64_t touched = 0;
+  for (uint64_t slot = 0; slot < nb && ok; slot++) {
+    if (last[slot] < 0) continue;
+    touched++;
+    payload((uint32_t)slot, (uint32_t)last[slot], vox.data());
+    for (int i = 0; i < 512 && ok; i++) {
+      ok = all[slot * 512 + i] == vox[i];
+      const bool bit = (masks[slot * 8 + i / 64] >> (i % 64)) & 1u;
+      ok = ok && bit == (vox[i] != 0);
+    }
+  }
+  const double bytes = (double)per * batches * (1024.0 + 4.0);
+  char buf[256];
+  std::snprintf(buf, sizeof(buf),
+                "{\"bricks_per_batch\": %d, \"batches\": %d, \"ms\": %.3f, \"mbricks_s\": %.3f, "
+                "\"gb_s\": %.2f, \"touched\": %llu, \"check\": %s}",
+                per, batches, ms, per * (double)batches / ms / 1e3, bytes / ms / 1e6,
+                (unsigned long long)touched, ok ? "true" : "false");
+  out = buf;
+  return ok;
+}
+
 bool parse3(const char *s, float v[3]) { return std::sscanf(s, "%f,%f,%f", &v[0], &v[1], &v[2]) == 3; }
 
 int usage() {
@@ -59,7 +158,8 @@ int usage() {
                "usage: meso_instance [--scene caves|terrain|terrain_caves|spheres] [--n N]\n"
                "  [--seed S] [--width W] [--height H] [--frames F] [--dirty D] [--ao K]\n"
                "  [--no-shadow] [--eye x,y,z] [--forward x,y,z] [--up x,y,z] [--step V]\n"
-               "  [--mesh culled|greedy] [--dump FILE] [--device I]\n");
+               "  [--mesh culled|greedy] [--dump FILE] [--device I]\n"
+               "  [--upload-bricks N --upload-batches B]\n");
   return 2;
 }
 
@@ -73,6 +173,7 @@ int main(int argc, char **argv) {
   float eye[3] = {-1, -1, -1}, fwd[3] = {1.0f, -0.3f, 0.6f}, up[3] = {0.0f, 1.0f, 0.0f};
   float step = 1.5f;
   bool shadow = true;
+  int32_t upload_bricks = 0, upload_batches = 0;
   for (int i = 1; i < argc; i++) {
     const std::string a = argv[i];
     const char *v = i + 1 < argc ? argv[i + 1] : nullptr;
@@ -94,6 +195,8 @@ int main(int argc, char **argv) {
     else if (a == "--step") step = (float)std::atof(v);
     else if (a == "--device") device = std::atoi(v);
     else if (a == "--dump") dump = v;
+    else if (a == "--upload-bricks") upload_bricks = std::atoi(v);
+    else if (a == "--upload-batches") upload_batches = std::atoi(v);
     else if (a == "--mesh") d.mesh_mode = std::string(v) == "culled" ? MESO_MESH_CULLED : MESO_MESH_GREEDY;
     else if (a == "--eye") { if (!parse3(v, eye)) return usage(); }
     else if (a == "--forward") { if (!parse3(v, fwd)) return usage(); }
@@ -169,7 +272,16 @@ int main(int argc, char **argv) {
   for (size_t i = 0; i < stats.size(); i++)
     std::printf("%s%" PRIu64, i ? ", " : "", stats[i].quads);
   const meso_camera &c = inst.FrameCamera();
-  std::printf("], \"eye\": [%.9g, %.9g, %.9g]}\n", c.eye[0], c.eye[1], c.eye[2]);
+  std::printf("], \"eye\": [%.9g, %.9g, %.9g]", c.eye[0], c.eye[1], c.eye[2]);
+  if (upload_bricks > 0 && upload_batches > 0 && dirty > 0) {
+    std::string up;
+    if (!run_upload(inst, upload_bricks, upload_batches, d.seed, up)) {
+      std::fprintf(stderr, "meso_instance: upload: %s\n", up.c_str());
+      return 1;
+    }
+    std::printf(", \"upload\": %s", up.c_str());
+  }
+  std::printf("}\n");
   std::fflush(stdout);
 
   if (!dump.empty()) {

@@ -115,3 +115,14 @@ def test_native_serving_driver_runs():
         assert 0 < out[path]["p50_ms"] <= out[path]["p99_ms"] <= out[path]["max_ms"] < 5000
         assert out[path]["mrays"] > 0
     assert out["served"]["launches"] >= 1
+
+
+def test_double_buffered_brick_upload():
+    """Host-edited bricks through the pinned double-buffered ring: every
+    touched brick's voxels and mask equal the last payload written to it."""
+    cmd = [BIN, "--scene", "caves", "--n", "256", "--frames", "1", "--dirty", "4", "--width", "64",
+           "--height", "48", "--upload-bricks", "3000", "--upload-batches", "5"]
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    up = json.loads(p.stdout.strip().splitlines()[-1])["upload"]
+    assert up["check"] is True and up["batches"] == 5 and up["touched"] > 3000
