@@ -236,6 +236,10 @@ static meso_status check_camera(const meso_camera *cam) {
              cam->height);
   MESO_CHECK(cam->vfov_deg > 0.0f && cam->vfov_deg < 180.0f, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
              "vfov %f outside (0,180)", (double)cam->vfov_deg);
+  for (int k = 0; k < 3; k++)
+    MESO_CHECK(std::isfinite(cam->eye[k]) && std::isfinite(cam->target[k]) &&
+                   std::isfinite(cam->up[k]),
+               MESO_ERR_ARGUMENT_OUT_OF_RANGE, "camera eye / target / up must be finite");
   float f[3] = {cam->target[0] - cam->eye[0], cam->target[1] - cam->eye[1],
                 cam->target[2] - cam->eye[2]};
   float c[3];
@@ -382,7 +386,19 @@ meso_status meso_stream_create(meso_context *ctx, meso_stream *out) {
 }
 meso_status meso_stream_destroy(meso_context *ctx, meso_stream stream) {
   MESO_CHECK(ctx && stream, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
-  MESO_HIP(hipStreamDestroy((hipStream_t)stream));
+  hipStream_t s = (hipStream_t)stream;
+  MESO_HIP(hipStreamSynchronize(s));
+  {
+    // the stream's raymarch counters go with it (a later stream may get the
+    // same handle; it then starts from a fresh, zeroed set)
+    std::lock_guard<std::mutex> lk(ctx->queue_mu);
+    auto it = ctx->band_queues.find(s);
+    if (it != ctx->band_queues.end()) {
+      hipFree(it->second);
+      ctx->band_queues.erase(it);
+    }
+  }
+  MESO_HIP(hipStreamDestroy(s));
   return MESO_OK;
 }
 meso_status meso_stream_sync(meso_context *ctx, meso_stream stream) {

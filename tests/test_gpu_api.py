@@ -202,6 +202,9 @@ def test_argument_errors(ctx):
     cam = R.camera((100, 100, 100), (32, 32, 32), (0, 0, 1), 60.0, 8, 8)
     bad_up = R.camera((0, 0, 100), (0, 0, 0), (0, 0, 1), 60.0, 8, 8)
     assert lib.meso_raymarch(ctx.h, sc.h, L.byref(bad_up), 0, None, L.byref(tg.out), None) == L.ERR_ARG
+    nan_eye = R.camera((float("nan"), 100, 100), (32, 32, 32), (0, 0, 1), 60.0, 8, 8)
+    assert lib.meso_raymarch(ctx.h, sc.h, L.byref(nan_eye), 0, None, L.byref(tg.out), None) == L.ERR_ARG
+    assert b"finite" in lib.meso_last_error_message()
     tg.out.index_bytes = 3
     assert lib.meso_raymarch(ctx.h, sc.h, L.byref(cam), 0, None, L.byref(tg.out), None) == L.ERR_ARG
     tg.out.index_bytes = 4
@@ -224,6 +227,22 @@ def test_argument_errors(ctx):
     assert b"slot" in lib.meso_last_error_message()
     assert lib.meso_buffer_upload(buf.h, m.ctypes.data, 1 << 30, 0) == L.ERR_ARG
     assert lib.meso_buffer_device_address(buf.h, 3) == 0
+    # a stream's raymarch counters go with the stream; a new stream renders the same bits
+    pin = L.TuneResult()
+    pin.size = C.sizeof(pin)
+    pin.variant, pin.tile_w, pin.tile_h, pin.grid = 5, 8, 8, 0  # persistent waves
+    L.check(lib.meso_autotune_pin_raymarch(ctx.h, sc.h, L.byref(cam), 0, L.byref(pin)))
+    for _ in range(2):
+        st = C.c_void_p()
+        L.check(lib.meso_stream_create(ctx.h, C.byref(st)))
+        tg2 = R.RayTargets(ctx, 8, 8)
+        L.check(lib.meso_raymarch(ctx.h, sc.h, L.byref(cam), 0, None, L.byref(tg2.out), st))
+        L.check(lib.meso_stream_sync(ctx.h, st))
+        L.check(lib.meso_stream_destroy(ctx.h, st))
+        R.raymarch(ctx, sc, cam, 0, None, tg)
+        ctx.sync()
+        for a, b in zip(tg.results(), tg2.results()):
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
     sc.destroy()
 
 
