@@ -149,7 +149,6 @@ enum : int {
 
 struct MeshLds {
   uint32_t M[kMaskWords];
-  uint32_t rec[32][kPlanes];  // open quad records [start u][plane]: first row | index << 5
   uint32_t scan[kMeshBlock / 64];
   uint32_t chunk;
   uint32_t base_lo, base_hi;
@@ -331,31 +330,51 @@ __device__ __forceinline__ uint32_t plane_count(const MeshLds &L, int t) {
   return count;
 }
 
-// Close the quads whose start bits are in `bits`: their last row is vend - 1.
-__device__ __forceinline__ void close_quads(const MeshLds &L, int t, int f, int s, uint32_t bits,
+// Start rows of the open quads, bit-sliced: bit u of row[k] is bit k of the
+// first row of the quad that starts at u (kept in registers, no table).
+struct StartRows {
+  uint32_t row[5];
+};
+
+__device__ __forceinline__ void record_starts(StartRows &r, uint32_t st, int v) {
+#pragma unroll
+  for (int k = 0; k < 5; k++) r.row[k] = (r.row[k] & ~st) | (((v >> k) & 1) ? st : 0u);
+}
+
+__device__ __forceinline__ int start_row(const StartRows &r, int u) {
+  int v = 0;
+#pragma unroll
+  for (int k = 0; k < 5; k++) v |= (int)((r.row[k] >> u) & 1u) << k;
+  return v;
+}
+
+// Close the quads whose start bits are in `bits` (their last row is
+// vend - 1), in ascending u, at out[base + count ...].
+__device__ __forceinline__ void close_quads(const StartRows &r, int f, int s, uint32_t bits,
                                             uint32_t open, uint32_t starts, int vend, uint32_t *out,
-                                            uint64_t base, uint64_t cap) {
+                                            uint64_t base, uint64_t cap, uint32_t &count) {
   const uint32_t cont = open & ~starts;
-  // two quads per trip: both records are read before either is used
-  while (bits) {
+  while (bits) {  // two quads per trip
     const int u1 = __ffs(bits) - 1;
     bits &= bits - 1;
     const bool two = bits != 0u;
     const int u2 = two ? __ffs(bits) - 1 : u1;
     bits &= two ? bits - 1 : bits;
-    const uint32_t r1 = L.rec[u1][t], r2 = L.rec[u2][t];
     const uint32_t a1 = u1 < 31 ? ~(cont >> (u1 + 1)) : 0u;  // zero where the segment ends
     const uint32_t a2 = u2 < 31 ? ~(cont >> (u2 + 1)) : 0u;
     const int w1 = 1 + (u1 < 31 ? (__ffs(a1) - 1) : 0);
     const int w2 = 1 + (u2 < 31 ? (__ffs(a2) - 1) : 0);
-    const int v1 = (int)(r1 & 31u), v2 = (int)(r2 & 31u);
-    const uint64_t k1 = base + (r1 >> 5), k2 = base + (r2 >> 5);
-    if (k1 < cap) out[k1] = pack_quad(f, s, u1, v1, w1, vend - v1);
-    if (two && k2 < cap) out[k2] = pack_quad(f, s, u2, v2, w2, vend - v2);
+    const int v1 = start_row(r, u1), v2 = start_row(r, u2);
+    const uint64_t k = base + count;
+    count += two ? 2u : 1u;
+    if (k < cap) out[k] = pack_quad(f, s, u1, v1, w1, vend - v1);
+    if (two && k + 1 < cap) out[k + 1] = pack_quad(f, s, u2, v2, w2, vend - v2);
   }
 }
 
-// Step 2b: write the quads of plane t at out[base + canonical index].
+// Step 2b: write the quads of plane t at out[base ...] in canonical order:
+// by the row where they end, then by start u (the order the sweep closes
+// them in; for 1x1 culled faces, row then u).
 template <bool GREEDY>
 __device__ __forceinline__ void plane_emit(MeshLds &L, int t, uint32_t *out, uint64_t base,
                                            uint64_t cap) {
@@ -381,32 +400,24 @@ __device__ __forceinline__ void plane_emit(MeshLds &L, int t, uint32_t *out, uin
     return;
   }
   uint32_t open = 0, starts = 0, ev_prev = 0;
+  StartRows r = {{0u, 0u, 0u, 0u, 0u}};
   for (int v = 0; v < 32; v++) {
     const uint32_t F = face_row(L, p, v);
     const uint32_t eh = L.M[p.eu + v * p.stride];
     if (v > 0) {
       const uint32_t dead = segment_fill(open & ~(F & ev_prev), open & ~starts) & open;
-      close_quads(L, t, f, s, starts & dead, open, starts, v, out, base, cap);
+      close_quads(r, f, s, starts & dead, open, starts, v, out, base, cap, count);
       open &= ~dead;
       starts &= ~dead;
     }
     const uint32_t avail = F & ~open;
-    uint32_t st = avail & ~((avail & eh) << 1);
+    const uint32_t st = avail & ~((avail & eh) << 1);
     open |= avail;
     starts |= st;
-    while (st) {
-      const int u1 = __ffs(st) - 1;
-      st &= st - 1;
-      const bool two = st != 0u;
-      const int u2 = two ? __ffs(st) - 1 : u1;
-      st &= two ? st - 1 : st;
-      L.rec[u1][t] = (uint32_t)v | (count << 5);
-      if (two) L.rec[u2][t] = (uint32_t)v | ((count + 1u) << 5);
-      count += two ? 2u : 1u;
-    }
+    record_starts(r, st, v);
     ev_prev = L.M[p.ev + v * p.stride];
   }
-  close_quads(L, t, f, s, starts, open, starts, 32, out, base, cap);
+  close_quads(r, f, s, starts, open, starts, 32, out, base, cap, count);
 }
 
 // exclusive scan of one value per lane over the workgroup; returns the total

@@ -904,10 +904,21 @@ static void plane_to_xyz(int na, int s, int u, int v, int *x, int *y, int *z) {
   else { *x = s; *y = u; *z = v; }
 }
 
+/* the plane's quads in output order: by last row (v + h - 1), then start u */
+typedef struct {
+  uint16_t key; /* (v + h - 1) * 32 + u */
+  uint32_t quad;
+} plane_quad;
+
+static int plane_quad_cmp(const void *a, const void *b) {
+  return (int)((const plane_quad *)a)->key - (int)((const plane_quad *)b)->key;
+}
+
 uint32_t mo_mesh_chunk(const void *vox, int es, int greedy, uint32_t *quads, uint32_t cap) {
   uint32_t count = 0;
   uint16_t mat[32][32];
   uint8_t open[32][32];
+  plane_quad pq[1024];
   for (int f = 0; f < 6; f++) {
     int na = face_axis(f), sg = face_sign(f);
     for (int s = 0; s < 32; s++) {
@@ -921,6 +932,7 @@ uint32_t mo_mesh_chunk(const void *vox, int es, int greedy, uint32_t *quads, uin
           open[v][u] = m != 0 && chunk_get(vox, es, nx, ny, nz) == 0;
           mat[v][u] = m;
         }
+      int np = 0;
       for (int v = 0; v < 32; v++)
         for (int u = 0; u < 32; u++) {
           if (!open[v][u]) continue;
@@ -941,9 +953,15 @@ uint32_t mo_mesh_chunk(const void *vox, int es, int greedy, uint32_t *quads, uin
             for (int k = 0; k < w; k++) open[v + j][u + k] = 0;
           int x, y, z;
           plane_to_xyz(na, s, u, v, &x, &y, &z);
-          if (count < cap && quads) quads[count] = mo_quad_pack(x, y, z, w, h, f);
-          count++;
+          pq[np].key = (uint16_t)((v + h - 1) * 32 + u);
+          pq[np].quad = mo_quad_pack(x, y, z, w, h, f);
+          np++;
         }
+      qsort(pq, (size_t)np, sizeof(plane_quad), plane_quad_cmp);  /* keys are distinct */
+      for (int i = 0; i < np; i++) {
+        if (count < cap && quads) quads[count] = pq[i].quad;
+        count++;
+      }
     }
   }
   return count;

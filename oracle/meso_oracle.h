@@ -212,7 +212,9 @@ void mo_ao_direction(uint32_t seed, uint32_t pixel, uint32_t sample, int face, f
 
 /* ---------------- meshing ---------------- */
 /* chunk of 32^3 voxels, x fastest; elem_size 1 (u8) or 2 (u16); 0 = empty.
- * Quads are emitted in canonical order: face, slice, v, u.
+ * Quads are emitted in canonical order: face, slice, then within the plane
+ * by the quad's last row (v + h - 1) and its start u (the order in which a
+ * row sweep closes them; for 1x1 culled quads this is v, u).
  * Returns the quad count (writes at most cap). */
 uint32_t mo_mesh_chunk(const void *vox, int elem_size, int greedy, uint32_t *quads,
                        uint32_t cap);
