@@ -149,6 +149,8 @@ enum : int {
 
 struct MeshLds {
   uint32_t M[kMaskWords];
+  uint32_t pex[kPlanes];  // culled: exclusive quad offset of each plane in the chunk
+  uint32_t occ[kPlanes];  // culled: bit v set when row v of the plane has a face
   uint32_t scan[kMeshBlock / 64];
   uint32_t chunk;
   uint32_t base_lo, base_hi;
@@ -300,14 +302,20 @@ __device__ __forceinline__ uint32_t segment_fill(uint32_t marks, uint32_t cont) 
   return up | dn;
 }
 
-// Step 2a: quad count of plane t.
+// Step 2a: quad count of plane t; culled meshing also returns the plane's
+// non-empty rows in `occ`.
 template <bool GREEDY>
-__device__ __forceinline__ uint32_t plane_count(const MeshLds &L, int t) {
+__device__ __forceinline__ uint32_t plane_count(const MeshLds &L, int t, uint32_t &occ) {
   const PlaneView p = plane_view(t >> 5, t & 31);
   uint32_t count = 0;
+  occ = 0;
   if (!GREEDY) {
 #pragma unroll 8
-    for (int v = 0; v < 32; v++) count += (uint32_t)__popc(face_row(L, p, v));
+    for (int v = 0; v < 32; v++) {
+      const uint32_t c = (uint32_t)__popc(face_row(L, p, v));
+      count += c;
+      occ |= (c != 0u ? 1u : 0u) << v;
+    }
     return count;
   }
   uint32_t open = 0, starts = 0, ev_prev = 0;
@@ -420,6 +428,80 @@ __device__ __forceinline__ void plane_emit(MeshLds &L, int t, uint32_t *out, uin
   close_quads(r, f, s, starts, open, starts, 32, out, base, cap, count);
 }
 
+// position of the r-th (0-based) set bit of x; x has more than r set bits
+__device__ __forceinline__ int select_bit(uint32_t x, uint32_t r) {
+  int pos = 0;
+#pragma unroll
+  for (int w = 16; w >= 1; w >>= 1) {
+    const uint32_t c = (uint32_t)__popc(x & ((1u << w) - 1u));
+    if (r >= c) {
+      r -= c;
+      x >>= w;
+      pos += w;
+    }
+  }
+  return pos;
+}
+
+// the last plane whose quad offset is <= q (offsets ascend and repeat over
+// empty planes, so for q < total this is the plane holding quad q)
+__device__ __forceinline__ int plane_of(const MeshLds &L, uint32_t q) {
+  int lo = 0, hi = kPlanes - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (L.pex[mid] <= q) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Culled emission balanced over the whole workgroup: thread i writes the
+// quads [total*i/n, total*(i+1)/n) of the chunk, in canonical order (plane,
+// row, u), so every thread loops the same number of times whatever the
+// spread of faces over the planes.  A thread finds its first face by a binary
+// search over the planes' offsets (pex) and a walk over the plane's non-empty
+// rows (occ); it crosses to the next plane by the same search.
+__device__ __forceinline__ void culled_emit_balanced(const MeshLds &L, uint32_t total,
+                                                     uint32_t *out, uint64_t base, uint64_t cap) {
+  const uint32_t t = threadIdx.x;
+  const uint32_t q0 = (uint32_t)(((uint64_t)total * t) / kMeshBlock);
+  const uint32_t q1 = (uint32_t)(((uint64_t)total * (t + 1u)) / kMeshBlock);
+  if (q0 >= q1) return;
+  int pl = plane_of(L, q0);
+  PlaneView p = plane_view(pl >> 5, pl & 31);
+  uint32_t r = q0 - L.pex[pl];
+  uint32_t rest = L.occ[pl];  // non-empty rows not yet entered
+  int v = __ffs(rest) - 1;
+  rest &= rest - 1u;
+  uint32_t row = face_row(L, p, v);
+  for (;;) {  // the row holding face r of the plane
+    const uint32_t c = (uint32_t)__popc(row);
+    if (r < c) break;
+    r -= c;
+    v = __ffs(rest) - 1;
+    rest &= rest - 1u;
+    row = face_row(L, p, v);
+  }
+  row &= ~((1u << select_bit(row, r)) - 1u);  // drop the faces before q0
+  for (uint32_t q = q0;;) {
+    const int u = __ffs(row) - 1;
+    row &= row - 1u;
+    const uint64_t k = base + q;
+    if (k < cap) out[k] = pack_quad(pl >> 5, pl & 31, u, v, 1, 1);
+    if (++q >= q1) break;
+    if (row == 0u) {  // next non-empty row, else the plane holding quad q
+      if (rest == 0u) {
+        pl = plane_of(L, q);
+        p = plane_view(pl >> 5, pl & 31);
+        rest = L.occ[pl];
+      }
+      v = __ffs(rest) - 1;
+      rest &= rest - 1u;
+      row = face_row(L, p, v);
+    }
+  }
+}
+
 // exclusive scan of one value per lane over the workgroup; returns the total
 __device__ __forceinline__ uint32_t block_exclusive_scan(MeshLds &L, uint32_t v, uint32_t &excl) {
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -482,7 +564,8 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_batched_kernel(
         cx = cy = cz = nc;  // outside the scene: every brick lookup reads empty
       build_masks<ES>(L, SceneRows<SRC>{ss.s, cx * 32, cy * 32, cz * 32});
     }
-    const uint32_t mine = t < kPlanes ? plane_count<GREEDY>(L, t) : 0u;
+    uint32_t occ = 0;
+    const uint32_t mine = t < kPlanes ? plane_count<GREEDY>(L, t, occ) : 0u;
     uint32_t excl = 0;
     const uint32_t chunk_total = block_exclusive_scan(L, mine, excl);
     if (t == 0) {
@@ -525,9 +608,18 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_batched_kernel(
       L.base_lo = (uint32_t)before;
       L.base_hi = (uint32_t)(before >> 32);
     }
+    if constexpr (!GREEDY)
+      if (t < kPlanes) {
+        L.pex[t] = excl;
+        L.occ[t] = occ;
+      }
     __syncthreads();
     const uint64_t base = ((uint64_t)L.base_hi << 32) | L.base_lo;
-    if (t < kPlanes) plane_emit<GREEDY>(L, t, quads, base + excl, capacity);
+    if constexpr (GREEDY) {
+      if (t < kPlanes) plane_emit<true>(L, t, quads, base + excl, capacity);
+    } else {
+      culled_emit_balanced(L, chunk_total, quads, base, capacity);
+    }
     __syncthreads();
   }
 }
@@ -539,11 +631,21 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_plain_kernel(const void *voxe
   const int t = (int)threadIdx.x;
   const ChunkAddr a{voxels, 0, 32, 1024};
   build_masks<ES>(L, a);
-  const uint32_t mine = t < kPlanes ? plane_count<GREEDY>(L, t) : 0u;
+  uint32_t occ = 0;
+  const uint32_t mine = t < kPlanes ? plane_count<GREEDY>(L, t, occ) : 0u;
   uint32_t excl = 0;
   const uint32_t chunk_total = block_exclusive_scan(L, mine, excl);
   if (t == 0) *count = chunk_total;
-  if (t < kPlanes) plane_emit<GREEDY>(L, t, quads, excl, capacity);
+  if constexpr (GREEDY) {
+    if (t < kPlanes) plane_emit<true>(L, t, quads, excl, capacity);
+  } else {
+    if (t < kPlanes) {
+      L.pex[t] = excl;
+      L.occ[t] = occ;
+    }
+    __syncthreads();
+    culled_emit_balanced(L, chunk_total, quads, 0, capacity);
+  }
 }
 
 template <int ES, bool GREEDY, int SRC>
