@@ -64,6 +64,36 @@ __global__ void __launch_bounds__(TW *TH) raymarch_kernel(DevScene s, DevCamera 
   }
 }
 
+// End of a persistent wave: add its counters, then leave.  The band counters
+// persist per stream: the last wave to leave (queue[128] counts leavers) sets
+// them back to zero, after every wave's last counter access has returned,
+// for the next launch.
+template <bool COUNT>
+__device__ __forceinline__ void persistent_leave(const RayOut &out, unsigned int *queue,
+                                                 const StepCounts &cnt, uint32_t n_hits,
+                                                 uint32_t n_rays, uint32_t n_shadow,
+                                                 uint32_t n_ao) {
+  const uint32_t lane = threadIdx.x;
+  if constexpr (COUNT) {
+    unsigned long long v[MESO_CNT_COUNT] = {cnt.n_vox, cnt.n_top, cnt.n_node, cnt.n_brickslot,
+                                            cnt.n_brickmask, n_hits, n_rays, n_shadow, n_ao,
+                                            cnt.n_coarse, cnt.n_iter};
+#pragma unroll
+    for (int c = 0; c < MESO_CNT_COUNT; c++) {
+      unsigned long long x = v[c];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+      if (lane == 0 && out.counters && x) atomicAdd(&out.counters[c], x);
+    }
+  }
+  if (lane == 0 && atomicAdd(&queue[128], 1u) == gridDim.x - 1u) {
+#pragma unroll
+    for (uint32_t b = 0; b < 8u; b++)
+      __hip_atomic_store(&queue[b * 16u], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&queue[128], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // Persistent waves: one 64-lane workgroup per wave slot of the device.  The
 // screen's PW x (64 / PW) packets (8x8, 16x4 or 32x2) are cut into 8 bands
 // of rows; the waves of XCD x
@@ -71,9 +101,7 @@ __global__ void __launch_bounds__(TW *TH) raymarch_kernel(DevScene s, DevCamera 
 // counter, then help the other bands in turn.  A wave leaves once every
 // counter has passed the end of its band, so the grid always drains; slots
 // freed by short (sky) packets are refilled at once instead of waiting for
-// the next workgroup dispatch.  The counters persist per stream: the last
-// wave to leave (queue[128] counts leavers) sets them back to zero, after
-// every wave's last counter access has returned, for the next launch.
+// the next workgroup dispatch.
 template <int KIND, bool COUNT, int PW>
 __global__ void __launch_bounds__(64) raymarch_persistent_kernel(
     DevScene s, DevCamera cam, uint32_t flags, DevShading sh, RayOut out, uint32_t packets_x,
@@ -109,24 +137,7 @@ __global__ void __launch_bounds__(64) raymarch_persistent_kernel(
       }
     }
   }
-  if constexpr (COUNT) {
-    unsigned long long v[MESO_CNT_COUNT] = {cnt.n_vox, cnt.n_top, cnt.n_node, cnt.n_brickslot,
-                                            cnt.n_brickmask, n_hits, n_rays, n_shadow, n_ao,
-                                            cnt.n_coarse, cnt.n_iter};
-#pragma unroll
-    for (int c = 0; c < MESO_CNT_COUNT; c++) {
-      unsigned long long x = v[c];
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-      if (lane == 0 && out.counters && x) atomicAdd(&out.counters[c], x);
-    }
-  }
-  if (lane == 0 && atomicAdd(&queue[128], 1u) == gridDim.x - 1u) {
-#pragma unroll
-    for (uint32_t b = 0; b < 8u; b++)
-      __hip_atomic_store(&queue[b * 16u], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&queue[128], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  persistent_leave<COUNT>(out, queue, cnt, n_hits, n_rays, n_shadow, n_ao);
 }
 
 template <int KIND, bool COUNT, int PW>

@@ -171,107 +171,127 @@ __device__ __forceinline__ bool occ_bit(const uint32_t *bits, uint32_t i) {
   return (bits[i >> 5] >> (i & 31u)) & 1u;
 }
 
+// Cached answers of the coarse levels for the cell a walk is in: packed keys
+// of the 128^3 block, 32^3 block and brick last looked up, and what they held.
+struct WalkCache {
+  int32_t c128, c32, cb;
+  bool e128, e32;
+  uint32_t node, slot;
+  uint64_t groups, sub;  // sub: 2^3-block bits of the current brick
+  __device__ __forceinline__ void reset() {
+    c128 = c32 = cb = -1;
+    e128 = e32 = false;
+    node = slot = kEmpty;
+    groups = sub = 0;
+  }
+};
+
+enum : int { kWalkOn = 0, kWalkHit = 1, kWalkOut = 2 };
+
+// One step of a walk: kWalkHit when the current cell is solid (material set),
+// kWalkOut when the entry t has reached tmax or the ray leaves the scene,
+// else kWalkOn after stepping to the next cell.  KIND: 1 dense u8 (voxel
+// steps only), 2 brick map, 3 two-level.  A brick scene steps at the coarsest
+// empty level holding the cell: 128^3 block, 32^3 block, brick, the 4^3 and
+// 2^3 blocks of a non-empty brick (from its mask, read once per brick), else
+// the voxel.
+template <int KIND, bool COUNT>
+__device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkState &w,
+                                         WalkCache &c, float tmax, uint16_t &material,
+                                         StepCounts &cnt) {
+  const int32_t n = s.n;
+  if (!(w.t < tmax)) return kWalkOut;
+  if (COUNT) cnt.n_iter++;
+  if constexpr (KIND == 1) {
+    if (COUNT) cnt.n_vox++;
+    const uint8_t v = s.dense[lin3(w.cell[0], w.cell[1], w.cell[2], n)];
+    if (v) {
+      material = v;
+      return kWalkHit;
+    }
+    return advance<false>(r, n, 0, w) ? kWalkOn : kWalkOut;
+  } else {
+    const uint32_t n128 = (uint32_t)(n + 127) >> 7, n32 = (uint32_t)(n + 31) >> 5;
+    // one flat pass over the levels (no nesting, so a wave whose lanes sit
+    // at different levels runs each piece once): refresh a level's cached
+    // answer when the cell left its block, then step at the coarsest empty
+    // level, or stop at a solid voxel
+    const uint32_t x = (uint32_t)w.cell[0], y = (uint32_t)w.cell[1], z = (uint32_t)w.cell[2];
+    const int32_t k128 = (int32_t)((x >> 7) | ((y >> 7) << 10) | ((z >> 7) << 20));
+    if (k128 != c.c128) {
+      c.c128 = k128;
+      c.c32 = -1;
+      c.cb = -1;
+      if constexpr (KIND == 3) {
+        if (COUNT) cnt.n_top++;
+        c.node = s.top[lin3u(x >> 7, y >> 7, z >> 7, n128)];
+        c.e128 = c.node == kEmpty;
+        c.groups = c.e128 ? 0ull : s.groups[c.node];
+      } else {
+        if (COUNT) cnt.n_coarse++;
+        c.e128 = !occ_bit(s.occ128, lin3u(x >> 7, y >> 7, z >> 7, n128));
+      }
+    }
+    const int32_t k32 = (int32_t)((x >> 5) | ((y >> 5) << 10) | ((z >> 5) << 20));
+    if (!c.e128 && k32 != c.c32) {
+      c.c32 = k32;
+      c.cb = -1;
+      if constexpr (KIND == 3) {
+        c.e32 = !((c.groups >> (((x >> 5) & 3u) + 4u * ((y >> 5) & 3u) + 16u * ((z >> 5) & 3u))) &
+                  1ull);
+      } else {
+        if (COUNT) cnt.n_coarse++;
+        c.e32 = !occ_bit(s.occ32, lin3u(x >> 5, y >> 5, z >> 5, n32));
+      }
+    }
+    const bool coarse = c.e128 || c.e32;
+    const int32_t kb = (int32_t)((x >> 3) | ((y >> 3) << 11) | ((z >> 3) << 22));
+    if (!coarse && kb != c.cb) {
+      c.cb = kb;
+      if constexpr (KIND == 3) {
+        if (COUNT) cnt.n_node++;
+        c.slot = s.slots[(uint64_t)c.node * 4096u +
+                         lin3u((x >> 3) & 15u, (y >> 3) & 15u, (z >> 3) & 15u, 16u)];
+      } else {
+        if (COUNT) cnt.n_brickslot++;
+        c.slot = s.slots[lin3u(x >> 3, y >> 3, z >> 3, (uint32_t)n >> 3)];
+      }
+      if (c.slot != kEmpty) {
+        if (COUNT) cnt.n_brickmask++;
+        c.sub = brick_blocks2(s.masks + (uint64_t)c.slot * 8u);
+      }
+    }
+    // inside a non-empty brick: step over its empty 4^3 and 2^3 blocks
+    const bool in_brick = !coarse && c.slot != kEmpty;
+    bool e4 = false, e2 = false;
+    if (in_brick) {
+      const uint32_t lx = x & 7u, ly = y & 7u, lz = z & 7u;
+      e4 = ((c.sub >> (2u * (lx >> 2) + 8u * (ly >> 2) + 32u * (lz >> 2))) & 0x00330033ull) == 0;
+      e2 = ((c.sub >> ((lx >> 1) + 4u * (ly >> 1) + 16u * (lz >> 1))) & 1ull) == 0;
+      if (!e4 && !e2) {
+        const uint64_t m = s.masks[(uint64_t)c.slot * 8u + lz];
+        if ((m >> (lx + 8u * ly)) & 1ull) {
+          material = s.voxels[(uint64_t)c.slot * 512u + lx + 8u * ly + 64u * lz];
+          return kWalkHit;
+        }
+      }
+    }
+    const int32_t sh =
+        c.e128 ? 7 : (c.e32 ? 5 : (!in_brick ? 3 : (e4 ? 2 : (e2 ? 1 : 0))));
+    return advance(r, n, sh, w) ? kWalkOn : kWalkOut;
+  }
+}
+
 // Walk until a solid voxel is entered (true), the ray leaves the scene or the
-// entry t reaches tmax (false).  KIND: 1 dense u8 (voxel steps only), 2 brick
-// map, 3 two-level.  Each step of a brick scene takes the coarsest empty
-// level holding the cell: 128^3 block, 32^3 block, brick, the 4^3 and 2^3
-// blocks of a non-empty brick (from its mask, read once per brick), else the
-// voxel.
+// entry t reaches tmax (false).
 template <int KIND, bool COUNT>
 __device__ __forceinline__ bool walk(const DevScene &s, const Ray &r, WalkState &w, float tmax,
                                      uint16_t &material, StepCounts &cnt) {
-  const int32_t n = s.n;
-  if constexpr (KIND == 1) {
-    for (;;) {
-      if (!(w.t < tmax)) return false;
-      if (COUNT) {
-        cnt.n_iter++;
-        cnt.n_vox++;
-      }
-      const uint8_t v = s.dense[lin3(w.cell[0], w.cell[1], w.cell[2], n)];
-      if (v) {
-        material = v;
-        return true;
-      }
-      if (!advance<false>(r, n, 0, w)) return false;
-    }
-  } else {
-    const uint32_t n128 = (uint32_t)(n + 127) >> 7, n32 = (uint32_t)(n + 31) >> 5;
-    int32_t c128 = -1, c32 = -1, cb = -1;  // packed block keys of the cached levels
-    bool e128 = false, e32 = false;
-    uint32_t node = kEmpty, slot = kEmpty;
-    uint64_t groups = 0, sub = 0;  // sub: 2^3-block bits of the current brick
-    for (;;) {
-      if (!(w.t < tmax)) return false;
-      if (COUNT) cnt.n_iter++;
-      // one flat pass over the levels (no nesting, so a wave whose lanes sit
-      // at different levels runs each piece once): refresh a level's cached
-      // answer when the cell left its block, then step at the coarsest empty
-      // level, or stop at a solid voxel
-      const uint32_t x = (uint32_t)w.cell[0], y = (uint32_t)w.cell[1], z = (uint32_t)w.cell[2];
-      const int32_t k128 = (int32_t)((x >> 7) | ((y >> 7) << 10) | ((z >> 7) << 20));
-      if (k128 != c128) {
-        c128 = k128;
-        c32 = -1;
-        cb = -1;
-        if constexpr (KIND == 3) {
-          if (COUNT) cnt.n_top++;
-          node = s.top[lin3u(x >> 7, y >> 7, z >> 7, n128)];
-          e128 = node == kEmpty;
-          groups = e128 ? 0ull : s.groups[node];
-        } else {
-          if (COUNT) cnt.n_coarse++;
-          e128 = !occ_bit(s.occ128, lin3u(x >> 7, y >> 7, z >> 7, n128));
-        }
-      }
-      const int32_t k32 = (int32_t)((x >> 5) | ((y >> 5) << 10) | ((z >> 5) << 20));
-      if (!e128 && k32 != c32) {
-        c32 = k32;
-        cb = -1;
-        if constexpr (KIND == 3) {
-          e32 = !((groups >> (((x >> 5) & 3u) + 4u * ((y >> 5) & 3u) + 16u * ((z >> 5) & 3u))) &
-                  1ull);
-        } else {
-          if (COUNT) cnt.n_coarse++;
-          e32 = !occ_bit(s.occ32, lin3u(x >> 5, y >> 5, z >> 5, n32));
-        }
-      }
-      const bool coarse = e128 || e32;
-      const int32_t kb = (int32_t)((x >> 3) | ((y >> 3) << 11) | ((z >> 3) << 22));
-      if (!coarse && kb != cb) {
-        cb = kb;
-        if constexpr (KIND == 3) {
-          if (COUNT) cnt.n_node++;
-          slot = s.slots[(uint64_t)node * 4096u +
-                         lin3u((x >> 3) & 15u, (y >> 3) & 15u, (z >> 3) & 15u, 16u)];
-        } else {
-          if (COUNT) cnt.n_brickslot++;
-          slot = s.slots[lin3u(x >> 3, y >> 3, z >> 3, (uint32_t)n >> 3)];
-        }
-        if (slot != kEmpty) {
-          if (COUNT) cnt.n_brickmask++;
-          sub = brick_blocks2(s.masks + (uint64_t)slot * 8u);
-        }
-      }
-      // inside a non-empty brick: step over its empty 4^3 and 2^3 blocks
-      const bool in_brick = !coarse && slot != kEmpty;
-      bool e4 = false, e2 = false;
-      if (in_brick) {
-        const uint32_t lx = x & 7u, ly = y & 7u, lz = z & 7u;
-        e4 = ((sub >> (2u * (lx >> 2) + 8u * (ly >> 2) + 32u * (lz >> 2))) & 0x00330033ull) == 0;
-        e2 = ((sub >> ((lx >> 1) + 4u * (ly >> 1) + 16u * (lz >> 1))) & 1ull) == 0;
-        if (!e4 && !e2) {
-          const uint64_t m = s.masks[(uint64_t)slot * 8u + lz];
-          if ((m >> (lx + 8u * ly)) & 1ull) {
-            material = s.voxels[(uint64_t)slot * 512u + lx + 8u * ly + 64u * lz];
-            return true;
-          }
-        }
-      }
-      const int32_t sh =
-          e128 ? 7 : (e32 ? 5 : (!in_brick ? 3 : (e4 ? 2 : (e2 ? 1 : 0))));
-      if (!advance(r, n, sh, w)) return false;
-    }
+  WalkCache c;
+  c.reset();
+  for (;;) {
+    const int st = walk_step<KIND, COUNT>(s, r, w, c, tmax, material, cnt);
+    if (st != kWalkOn) return st == kWalkHit;
   }
 }
 
@@ -310,18 +330,13 @@ struct Hit {
   uint16_t material;
 };
 
-// Clip to [0,n]^3, find the entry cell and walk.
-template <int KIND, bool COUNT>
-__device__ __forceinline__ bool trace(const DevScene &s, const float o[3], const float d[3],
-                                      Hit &hit, StepCounts &cnt) {
-  Ray r;
+// Set up the ray o + t d, clip it to [0,n]^3 and find its entry cell; false
+// when it misses the scene.
+__device__ __forceinline__ bool trace_begin(const DevScene &s, const float o[3], const float d[3],
+                                            Ray &r, WalkState &w) {
   ray_setup(r, o, d);
   const int32_t n = s.n;
   const float fn = (float)n;
-  hit.index = -1;
-  hit.face = kFaceMiss;
-  hit.t = __builtin_inff();
-  hit.material = 0;
   float t_enter = 0.0f, t_exit = __builtin_inff();
   int ea = -1;
 #pragma unroll
@@ -339,7 +354,6 @@ __device__ __forceinline__ bool trace(const DevScene &s, const float o[3], const
     if (hi < t_exit) t_exit = hi;
   }
   if (t_enter > t_exit) return false;
-  WalkState w;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     if (a == ea) {
@@ -359,6 +373,21 @@ __device__ __forceinline__ bool trace(const DevScene &s, const float o[3], const
   for (int a = 0; a < 3; a++)
     if (a == ea) est = r.step[a];
   w.face = ea >= 0 ? entry_face(ea, est) : kFaceInside;
+  return true;
+}
+
+// Clip to [0,n]^3, find the entry cell and walk.
+template <int KIND, bool COUNT>
+__device__ __forceinline__ bool trace(const DevScene &s, const float o[3], const float d[3],
+                                      Hit &hit, StepCounts &cnt) {
+  hit.index = -1;
+  hit.face = kFaceMiss;
+  hit.t = __builtin_inff();
+  hit.material = 0;
+  Ray r;
+  WalkState w;
+  if (!trace_begin(s, o, d, r, w)) return false;
+  const int32_t n = s.n;
   uint16_t m = 0;
   if (!walk<KIND, COUNT>(s, r, w, __builtin_inff(), m, cnt)) return false;
   hit.index = lin3(w.cell[0], w.cell[1], w.cell[2], n);
@@ -371,21 +400,31 @@ __device__ __forceinline__ bool trace(const DevScene &s, const float o[3], const
   return true;
 }
 
-template <int KIND, bool COUNT>
-__device__ __forceinline__ bool occluded(const DevScene &s, const float o[3], const float d[3],
-                                         const int32_t start[3], float tmax, StepCounts &cnt) {
-  const int32_t n = s.n;
-#pragma unroll
-  for (int a = 0; a < 3; a++)
-    if (start[a] < 0 || start[a] >= n) return false;
-  Ray r;
+// true when a secondary ray can start in cell `start` (inside the scene);
+// outside it, the ray is unoccluded
+__device__ __forceinline__ bool start_inside(const DevScene &s, const int32_t start[3]) {
+  return start[0] >= 0 && start[0] < s.n && start[1] >= 0 && start[1] < s.n && start[2] >= 0 &&
+         start[2] < s.n;
+}
+
+// A secondary ray o + t d that walks from cell `start` at t = 0.
+__device__ __forceinline__ void occluded_begin(const float o[3], const float d[3],
+                                               const int32_t start[3], Ray &r, WalkState &w) {
   ray_setup(r, o, d);
-  WalkState w;
   w.cell[0] = start[0];
   w.cell[1] = start[1];
   w.cell[2] = start[2];
   w.t = 0.0f;
   w.face = kFaceInside;
+}
+
+template <int KIND, bool COUNT>
+__device__ __forceinline__ bool occluded(const DevScene &s, const float o[3], const float d[3],
+                                         const int32_t start[3], float tmax, StepCounts &cnt) {
+  if (!start_inside(s, start)) return false;
+  Ray r;
+  WalkState w;
+  occluded_begin(o, d, start, r, w);
   uint16_t m;
   return walk<KIND, COUNT>(s, r, w, tmax, m, cnt);
 }
@@ -446,48 +485,47 @@ struct PixelResult {
   float shade;
 };
 
-// Secondary rays of a primary hit (voxel `cell`, entry face `face` < 6, at t
-// along o + t d): the shadow ray toward the light when the face is lit, the
-// AO rays, and the shade (0.25 + 0.75 * lambert * visible) * (0.5 + 0.5 *
-// unoccluded fraction).  With neither flag set it traces nothing.
-template <int KIND, bool COUNT>
-__device__ __forceinline__ float shade_hit(const DevScene &s, uint32_t flags, const DevShading &sh,
-                                           uint32_t pix, const float o[3], const float d[3],
-                                           int32_t face, float t, const int32_t cell[3],
-                                           StepCounts &cnt, uint32_t &n_shadow, uint32_t &n_ao) {
+// Where the secondary rays of a primary hit start: the hit point p = o + t d
+// on entry face `face` (< 6) of voxel `cell`, the cell in front of that face,
+// and the face's Lambert term max(0, n . light).
+struct Secondary {
+  float p[3];
+  int32_t start[3];
+  float lambert;
+};
+
+__device__ __forceinline__ void secondary_setup(const DevShading &sh, const float o[3],
+                                                const float d[3], int32_t face, float t,
+                                                const int32_t cell[3], Secondary &sc) {
   const int na = face_axis(face);
   const int sg = (face & 1) ? -1 : 1;
   float lightc = sh.light[0];
-  int32_t start[3] = {cell[0], cell[1], cell[2]};
+  sc.start[0] = cell[0];
+  sc.start[1] = cell[1];
+  sc.start[2] = cell[2];
 #pragma unroll
   for (int k = 0; k < 3; k++)
     if (k == na) {
       lightc = sh.light[k];
-      start[k] += sg;
+      sc.start[k] += sg;
     }
   float lambert = sg > 0 ? lightc : -lightc;
   if (lambert < 0.0f) lambert = 0.0f;
-  float p[3];
+  sc.lambert = lambert;
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     float dt = d[i] * t;
-    p[i] = o[i] + dt;
+    sc.p[i] = o[i] + dt;
   }
-  float vis = 1.0f;
-  if ((flags & MESO_RAY_SHADOW) && lambert > 0.0f) {
-    n_shadow++;
-    if (occluded<KIND, COUNT>(s, p, sh.light, start, __builtin_inff(), cnt)) vis = 0.0f;
-  }
+}
+
+// (0.25 + 0.75 * lambert * visible) * (0.5 + 0.5 * unoccluded / samples),
+// the second factor only with AO rays
+__device__ __forceinline__ float shade_value(float lambert, float vis, bool ao_on, int32_t unocc,
+                                             int32_t samples) {
   float ao = 1.0f;
-  if (flags & MESO_RAY_AO) {
-    int32_t unocc = 0;
-    for (int32_t k = 0; k < sh.ao_samples; k++) {
-      float dir[3];
-      ao_direction(sh.ao_seed, pix, (uint32_t)k, face, dir);
-      n_ao++;
-      if (!occluded<KIND, COUNT>(s, p, dir, start, sh.ao_radius, cnt)) unocc++;
-    }
-    float frac = (float)unocc / (float)sh.ao_samples;
+  if (ao_on) {
+    float frac = (float)unocc / (float)samples;
     frac = 0.5f * frac;
     ao = 0.5f + frac;
   }
@@ -496,6 +534,33 @@ __device__ __forceinline__ float shade_hit(const DevScene &s, uint32_t flags, co
   float shade = 0.25f + l;
   shade = shade * ao;
   return shade;
+}
+
+// Secondary rays of a primary hit (voxel `cell`, entry face `face` < 6, at t
+// along o + t d): the shadow ray toward the light when the face is lit, the
+// AO rays, and the shade.  With neither flag set it traces nothing.
+template <int KIND, bool COUNT>
+__device__ __forceinline__ float shade_hit(const DevScene &s, uint32_t flags, const DevShading &sh,
+                                           uint32_t pix, const float o[3], const float d[3],
+                                           int32_t face, float t, const int32_t cell[3],
+                                           StepCounts &cnt, uint32_t &n_shadow, uint32_t &n_ao) {
+  Secondary sc;
+  secondary_setup(sh, o, d, face, t, cell, sc);
+  float vis = 1.0f;
+  if ((flags & MESO_RAY_SHADOW) && sc.lambert > 0.0f) {
+    n_shadow++;
+    if (occluded<KIND, COUNT>(s, sc.p, sh.light, sc.start, __builtin_inff(), cnt)) vis = 0.0f;
+  }
+  int32_t unocc = 0;
+  if (flags & MESO_RAY_AO) {
+    for (int32_t k = 0; k < sh.ao_samples; k++) {
+      float dir[3];
+      ao_direction(sh.ao_seed, pix, (uint32_t)k, face, dir);
+      n_ao++;
+      if (!occluded<KIND, COUNT>(s, sc.p, dir, sc.start, sh.ao_radius, cnt)) unocc++;
+    }
+  }
+  return shade_value(sc.lambert, vis, (flags & MESO_RAY_AO) != 0, unocc, sh.ao_samples);
 }
 
 // true when a primary hit on `face` needs secondary rays
