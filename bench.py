@@ -263,7 +263,23 @@ def cpu_leg(W, seconds_cap=30.0):
                       "median of %d runs on %d threads; scene build %.2fs" % (len(times), cores, gen_s),
                single_thread_mrays=(c1.rays + c1.shadow_rays) / st / 1e6)
     out.update(cpu_extras(O, cores))
+    out["chunk_streaming"] = cpu_pool_sample(O, sc)
     return out
+
+
+def cpu_pool_sample(O, sc):
+    """the streaming leg's sample updates (512^3 heightfield, 4,096 slots,
+    radius 16) on the oracle's chunk pool, one thread"""
+    from mesoengine_amd import streaming as S
+    pool = O.OraclePool(sc, O.pool_params(greedy=True, **S.SAMPLE))
+    loads = 0
+    t = time.perf_counter()
+    for cam, fwd in S.SAMPLE_PATH:
+        loads += pool.update(cam, fwd)[1]
+    dt = time.perf_counter() - t
+    return dict(ms=round(dt * 1e3, 1), loads=loads, mvox_per_s=round(loads * 32768 / dt / 1e6, 1),
+                threads=1, sample="%d updates of the 512^3 heightfield chunk pool "
+                                  "(rank, evict, greedy-mesh the loads)" % len(S.SAMPLE_PATH))
 
 
 def _median_time(fn, runs=5, cap_s=6.0):
@@ -362,12 +378,18 @@ def main():
                       serve_requests=args.serve_requests, budget_bytes=args.budget_mb << 20,
                       log=log)
     upload = None if args.no_frame else upload_leg()
+    streaming = None
+    if not args.no_frame:
+        from mesoengine_amd import streaming as S
+        streaming = S.run(ctx, log=log)
     cpu = None if args.no_cpu else cpu_leg(W)
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tp):
+        # the PMC traffic of the launch shape the autotuner picked here
+        shape = {5: "p8x8", 6: "p16x4", 7: "p32x2"}.get(rm["tune"]["variant"], "tiled")
         with open(tp) as f:
-            k = json.load(f).get("kernels", {}).get("raymarch_brickmap_terrain512")
+            k = json.load(f).get("kernels", {}).get("raymarch_brickmap_terrain512_" + shape)
             traffic = k["hbm_bytes"] if k else None
     line = {
         "metric": "Mrays/s and p99 latency at fixed QPS; Mvoxels meshed/s; batched vs plain",
@@ -398,6 +420,7 @@ def main():
         "serving": serve,
         "frame": frame,
         "brick_upload": upload,
+        "chunk_streaming": streaming,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -381,6 +381,79 @@ meso_status meso_autotune_load(meso_context *ctx, const char *path);
 meso_status meso_autotune_save(meso_context *ctx, const char *path);
 
 /* ------------------------------------------------------------------ */
+/* chunk pool: the chunks around a camera, ranked by importance and kept */
+/* meshed in a fixed number of slots                                     */
+/* ------------------------------------------------------------------ */
+/* Replaces the reference's chunk streaming: FChunkManage::UpdateChunks /
+ * UpdateLoadingQueue (Runtimes/Voxel/Chunk/ChunkManager.h:160-330), the
+ * fixed-capacity FChunkPool with EChunkOverrideMode::FindMin replacement
+ * (ChunkPool.h:446-560) and FChunkManageHelper's importance and desired set
+ * (ChunkManagerHelper.h:21-148).  Chunks are the 32^3 chunks of a resident
+ * brick map or two-level scene (index x + nc*(y + nc*z), nc = ceil(n/32));
+ * a slot holds the packed quads of one chunk, meshed from the scene. */
+typedef struct meso_chunk_pool meso_chunk_pool;
+typedef struct meso_pool_desc {
+  uint32_t size;           /* sizeof(meso_pool_desc) */
+  int32_t capacity;        /* slots, 1..2^22 (FVoxelSceneConfig::MaxChunkCount: 16384) */
+  int32_t quads_per_slot;  /* quad capacity of a slot, 1..98304; longer meshes are cut */
+  int32_t forward_radius;  /* ViewForwardLoadChunkSize, 1..64 chunks (24) */
+  int32_t backward_radius; /* ViewBackwardLoadChunkSize, 0..forward_radius (6) */
+  float view_angle;        /* ViewChunkAngle, degrees in (0, 360] (120) */
+  int32_t max_loads;       /* chunks loaded per update, 1..capacity
+                              (MaxSyncedLoadChunkCount + MaxUnsyncedLoadChunkCount: 256) */
+  uint32_t mesh_mode;      /* MESO_MESH_CULLED or MESO_MESH_GREEDY */
+} meso_pool_desc;
+typedef struct meso_pool_stats {
+  uint32_t size;          /* sizeof(meso_pool_stats) */
+  uint32_t stamp;         /* updates so far (the frame stamp; wraps at 2^32) */
+  uint32_t n_candidates;  /* last update: desired non-empty chunks not resident */
+  uint32_t n_loads;       /* last update: chunks loaded and meshed */
+  uint32_t n_evicted;     /* last update: resident chunks they replaced */
+  uint32_t n_resident;    /* slots holding a chunk */
+  uint32_t n_truncated;   /* resident chunks with more quads than a slot holds */
+  uint32_t pad;
+  uint64_t n_quads;       /* quads of the resident chunks (full counts) */
+  uint64_t device_bytes;  /* device memory of the pool */
+} meso_pool_stats;
+typedef struct meso_pool_arrays {
+  uint32_t size;                /* sizeof(meso_pool_arrays) */
+  int32_t chunks_per_axis;      /* nc */
+  const uint32_t *lookup;       /* nc^3: slot of each chunk, or 0xFFFFFFFF */
+  const uint32_t *slot_chunk;   /* capacity: chunk index held, or 0xFFFFFFFF (free) */
+  const uint32_t *slot_stamp;   /* capacity: stamp of the update that loaded it */
+  const uint32_t *slot_count;   /* capacity: the chunk's quad count (may exceed a slot) */
+  const uint32_t *quads;        /* capacity * quads_per_slot packed quads, chunk-local */
+  const int32_t *load_xyz;      /* the last update's loads: max_loads x (x, y, z) */
+  const uint32_t *load_slot;    /* and their slots; n_loads of them are valid */
+} meso_pool_arrays;
+/* Creates an empty pool over a resident scene (the scene must outlive it). */
+meso_status meso_chunk_pool_create(meso_context *ctx, meso_scene *scene, const meso_pool_desc *desc,
+                                   meso_chunk_pool **out);
+meso_status meso_chunk_pool_destroy(meso_chunk_pool *pool);
+/* One update for a camera in chunk camera_chunk looking along forward (any
+ * length; normalised here).  Candidates are the desired chunks (inside the
+ * forward sphere; the 3^3 core, or nearer than a radius that blends from the
+ * forward radius inside the view cone to the backward radius outside) that
+ * are inside the scene, hold a non-empty brick and are not resident, ranked
+ * by CalculateChunkImportance (descending), then chunk index.  Slots rank
+ * free first, then by the importance of their chunk under this camera
+ * (ascending).  Candidate i takes slot i while i < max_loads and the slot is
+ * free or its chunk strictly less important; the candidates left wait for
+ * later updates (the reference's re-queue).  The loaded chunks are meshed
+ * into their slots.  Runs on `stream` without a host wait.  Calls on one
+ * pool must be ordered (one stream, or synchronised). */
+meso_status meso_chunk_pool_update(meso_chunk_pool *pool, const int32_t camera_chunk[3],
+                                   const float forward[3], meso_stream stream);
+/* Remeshes the resident chunks of a dirty list (device chunk coordinates,
+ * e.g. after meso_scene_edit_chunks) into their slots; others are skipped. */
+meso_status meso_chunk_pool_remesh(meso_chunk_pool *pool, const int32_t *d_chunk_xyz,
+                                   int32_t n_chunks, meso_stream stream);
+/* Counts over the pool; waits for `stream`. */
+meso_status meso_chunk_pool_stats(meso_chunk_pool *pool, meso_pool_stats *out, meso_stream stream);
+/* Device arrays of the pool, for consumers (valid until destroy). */
+meso_status meso_chunk_pool_arrays(meso_chunk_pool *pool, meso_pool_arrays *out);
+
+/* ------------------------------------------------------------------ */
 /* serving: continuous batching of raymarch requests over resident       */
 /* scenes, admitted by an HBM budget (the GPU counterpart of the         */
 /* streaming loop FChunkManage::UpdateLoadingQueue, ChunkManager.h:211-400, */

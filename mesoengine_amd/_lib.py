@@ -90,6 +90,26 @@ class HostOutputs(C.Structure):
                 ("face", C.c_void_p), ("t", C.c_void_p), ("shade", C.c_void_p)]
 
 
+class PoolDesc(C.Structure):
+    _fields_ = [("size", C.c_uint32), ("capacity", C.c_int32), ("quads_per_slot", C.c_int32),
+                ("forward_radius", C.c_int32), ("backward_radius", C.c_int32),
+                ("view_angle", C.c_float), ("max_loads", C.c_int32), ("mesh_mode", C.c_uint32)]
+
+
+class PoolStats(C.Structure):
+    _fields_ = [("size", C.c_uint32), ("stamp", C.c_uint32), ("n_candidates", C.c_uint32),
+                ("n_loads", C.c_uint32), ("n_evicted", C.c_uint32), ("n_resident", C.c_uint32),
+                ("n_truncated", C.c_uint32), ("pad", C.c_uint32), ("n_quads", C.c_uint64),
+                ("device_bytes", C.c_uint64)]
+
+
+class PoolArrays(C.Structure):
+    _fields_ = [("size", C.c_uint32), ("chunks_per_axis", C.c_int32), ("lookup", C.c_void_p),
+                ("slot_chunk", C.c_void_p), ("slot_stamp", C.c_void_p),
+                ("slot_count", C.c_void_p), ("quads", C.c_void_p), ("load_xyz", C.c_void_p),
+                ("load_slot", C.c_void_p)]
+
+
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
 _SIGS = {
@@ -155,6 +175,12 @@ _SIGS = {
     "meso_serve_take": (C.c_int32, [P, C.c_uint64, P]),
     "meso_serve_stats_get": (C.c_int32, [P, P]),
     "meso_serve_stop": (C.c_int32, [P]),
+    "meso_chunk_pool_create": (C.c_int32, [P, P, P, PP]),
+    "meso_chunk_pool_destroy": (C.c_int32, [P]),
+    "meso_chunk_pool_update": (C.c_int32, [P, P, P, P]),
+    "meso_chunk_pool_remesh": (C.c_int32, [P, P, C.c_int32, P]),
+    "meso_chunk_pool_stats": (C.c_int32, [P, P, P]),
+    "meso_chunk_pool_arrays": (C.c_int32, [P, P]),
 }
 EXPORTED = sorted(_SIGS)
 

@@ -624,6 +624,54 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_batched_kernel(
   }
 }
 
+// Chunks of a resident scene meshed into fixed slots: chunk c of the list
+// (coordinates xyz[3c]) writes its quads at slots[c] * per_slot, at most
+// per_slot of them, and its full count to slot_count[slots[c]].  The list
+// length is read on the device (*n_dev, at most max_n), so that a producer
+// kernel can size it.  Workgroups take chunks from ticket[0]; the last workgroup to leave
+// (ticket[1] counts leavers) sets both back to zero for the next launch.
+template <bool GREEDY, int SRC>
+__global__ void __launch_bounds__(kMeshBlock) mesh_slotted_kernel(
+    DevScene sc, const int32_t *xyz, const uint32_t *slots, const uint32_t *n_dev, uint32_t max_n,
+    uint32_t *slot_count, uint32_t *quads, uint32_t per_slot, unsigned int *ticket) {
+  __shared__ MeshLds L;
+  const int t = (int)threadIdx.x;
+  const uint32_t n_chunks = *n_dev < max_n ? *n_dev : max_n;
+  const int32_t nc = (sc.n + 31) >> 5;  // a partial last chunk counts
+  for (;;) {
+    if (t == 0) L.chunk = atomicAdd(&ticket[0], 1u);
+    __syncthreads();
+    const uint32_t c = L.chunk;
+    if (c >= n_chunks) break;  // uniform: every wave leaves
+    int32_t cx = xyz[3 * c], cy = xyz[3 * c + 1], cz = xyz[3 * c + 2];
+    if (cx < 0 || cy < 0 || cz < 0 || cx >= nc || cy >= nc || cz >= nc)
+      cx = cy = cz = nc;  // outside the scene: every brick lookup reads empty
+    build_masks<2>(L, SceneRows<SRC>{sc, cx * 32, cy * 32, cz * 32});
+    uint32_t occ = 0;
+    const uint32_t mine = t < kPlanes ? plane_count<GREEDY>(L, t, occ) : 0u;
+    uint32_t excl = 0;
+    const uint32_t chunk_total = block_exclusive_scan(L, mine, excl);
+    const uint32_t slot = slots[c];
+    const uint64_t base = (uint64_t)slot * per_slot, cap = base + per_slot;
+    if (t == 0) slot_count[slot] = chunk_total;
+    if constexpr (GREEDY) {
+      if (t < kPlanes) plane_emit<true>(L, t, quads, base + excl, cap);
+    } else {
+      if (t < kPlanes) {
+        L.pex[t] = excl;
+        L.occ[t] = occ;
+      }
+      __syncthreads();
+      culled_emit_balanced(L, chunk_total, quads, base, cap);
+    }
+    __syncthreads();
+  }
+  if (t == 0 && atomicAdd(&ticket[1], 1u) == gridDim.x - 1u) {
+    __hip_atomic_store(&ticket[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&ticket[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 template <int ES, bool GREEDY>
 __global__ void __launch_bounds__(kMeshBlock) mesh_plain_kernel(const void *voxels, uint32_t *count,
                                                                 uint32_t *quads, uint32_t capacity) {
@@ -733,6 +781,42 @@ static meso_status launch_mesh_scene(meso_context *ctx, const meso_scene *scene,
                                           d_total, d_work, stream, nullptr)
            : launch_batched_t<2, false, 3>(ctx, none, ss, n, d_counts, d_prefix, d_quads,
                                            capacity, d_total, d_work, stream, nullptr);
+}
+
+template <bool GREEDY, int SRC>
+static meso_status launch_slotted_t(meso_context *ctx, const meso_scene *scene,
+                                    const int32_t *xyz, const uint32_t *slots,
+                                    const uint32_t *n_dev, int32_t max_chunks,
+                                    uint32_t *slot_count, uint32_t *quads, uint32_t per_slot,
+                                    unsigned int *ticket, hipStream_t stream) {
+  int per_cu = 0;
+  MESO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, mesh_slotted_kernel<GREEDY, SRC>, kMeshBlock, 0));
+  if (per_cu < 1) per_cu = 1;
+  int grid = per_cu * ctx->props.multiProcessorCount;
+  if (grid > max_chunks) grid = max_chunks;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL((mesh_slotted_kernel<GREEDY, SRC>), dim3(grid), dim3(kMeshBlock), 0, stream,
+                     scene->dev(), xyz, slots, n_dev, (uint32_t)max_chunks, slot_count, quads,
+                     per_slot, ticket);
+  MESO_HIP(hipGetLastError());
+  return MESO_OK;
+}
+
+meso_status launch_mesh_slotted(meso_context *ctx, const meso_scene *scene, const int32_t *xyz,
+                                const uint32_t *slots, const uint32_t *n_dev, int32_t max_chunks,
+                                uint32_t mode, uint32_t *slot_count, uint32_t *quads,
+                                uint32_t per_slot, unsigned int *ticket, hipStream_t stream) {
+  const bool g = mode == MESO_MESH_GREEDY;
+  if (scene->kind == MESO_SCENE_BRICKMAP_U16)
+    return g ? launch_slotted_t<true, 2>(ctx, scene, xyz, slots, n_dev, max_chunks, slot_count,
+                                         quads, per_slot, ticket, stream)
+             : launch_slotted_t<false, 2>(ctx, scene, xyz, slots, n_dev, max_chunks, slot_count,
+                                          quads, per_slot, ticket, stream);
+  return g ? launch_slotted_t<true, 3>(ctx, scene, xyz, slots, n_dev, max_chunks, slot_count,
+                                       quads, per_slot, ticket, stream)
+           : launch_slotted_t<false, 3>(ctx, scene, xyz, slots, n_dev, max_chunks, slot_count,
+                                        quads, per_slot, ticket, stream);
 }
 
 template <int ES, bool GREEDY>

@@ -170,6 +170,13 @@ meso_status launch_mesh_batched(meso_context *ctx, const meso_chunk_source *src,
                                 uint64_t capacity, uint64_t *d_total, void *d_work,
                                 hipStream_t stream, const MeshLaunch *shape);
 MeshLaunch default_mesh_launch(meso_context *ctx);
+// meshes the first *n_dev chunks of a scene chunk list into fixed slots
+// (quads at slots[c] * per_slot); ticket: 2 counters, zero before the first
+// launch, left at zero by every launch
+meso_status launch_mesh_slotted(meso_context *ctx, const meso_scene *scene, const int32_t *xyz,
+                                const uint32_t *slots, const uint32_t *n_dev, int32_t max_chunks,
+                                uint32_t mode, uint32_t *slot_count, uint32_t *quads,
+                                uint32_t per_slot, unsigned int *ticket, hipStream_t stream);
 
 // host PCG32 / SplitMix64 (same published algorithms as the generators use)
 uint64_t splitmix64(uint64_t *s);

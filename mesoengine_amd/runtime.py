@@ -349,3 +349,60 @@ def mesh_scene_chunks(ctx, scene, xyz_buf, n_chunks, mode, mt, stream=None):
     check(ctx.lib.meso_mesh_scene_chunks(ctx.h, scene.h, xyz_buf.ptr, n_chunks, mode,
                                          mt.counts.ptr, mt.prefix.ptr, mt.quads.ptr, mt.capacity,
                                          mt.total.ptr, mt.work.ptr, stream))
+
+
+class ChunkPool:
+    """The chunks around a camera, ranked by importance and kept meshed in
+    fixed slots (meso_chunk_pool_*)."""
+
+    def __init__(self, ctx, scene, capacity=16384, quads_per_slot=8192, forward_radius=24,
+                 backward_radius=6, view_angle=120.0, max_loads=256, mode=L.MESH_GREEDY):
+        self.ctx, self.scene = ctx, scene
+        self.desc = L.PoolDesc(C.sizeof(L.PoolDesc), capacity, quads_per_slot, forward_radius,
+                               backward_radius, view_angle, max_loads, mode)
+        h = C.c_void_p()
+        check(ctx.lib.meso_chunk_pool_create(ctx.h, scene.h, byref(self.desc), C.byref(h)))
+        self.h = h
+        a = L.PoolArrays()
+        a.size = C.sizeof(a)
+        check(ctx.lib.meso_chunk_pool_arrays(self.h, byref(a)))
+        self.arrays = a
+        self.nc = a.chunks_per_axis
+
+    def update(self, cam_chunk, forward, stream=None):
+        cam = (C.c_int32 * 3)(*[int(v) for v in cam_chunk])
+        fwd = (C.c_float * 3)(*[float(v) for v in forward])
+        check(self.ctx.lib.meso_chunk_pool_update(self.h, C.cast(cam, C.c_void_p),
+                                                  C.cast(fwd, C.c_void_p), stream))
+
+    def remesh(self, xyz_buf, n_chunks, stream=None):
+        check(self.ctx.lib.meso_chunk_pool_remesh(self.h, xyz_buf.ptr, n_chunks, stream))
+
+    def stats(self, stream=None):
+        s = L.PoolStats()
+        s.size = C.sizeof(s)
+        check(self.ctx.lib.meso_chunk_pool_stats(self.h, byref(s), stream))
+        return {k: getattr(s, k) for k, _ in L.PoolStats._fields_ if k not in ("size", "pad")}
+
+    def _d2h(self, ptr, dtype, count):
+        out = np.empty(count, dtype)
+        if count:
+            check(self.ctx.lib.meso_memcpy_d2h(self.ctx.h, out.ctypes.data, ptr, out.nbytes, None))
+        return out
+
+    def state(self):
+        """The pool's arrays, downloaded (waits for the context stream)."""
+        cap, q = self.desc.capacity, self.desc.quads_per_slot
+        a = self.arrays
+        return {"lookup": self._d2h(a.lookup, np.uint32, self.nc ** 3),
+                "slot_chunk": self._d2h(a.slot_chunk, np.uint32, cap),
+                "slot_stamp": self._d2h(a.slot_stamp, np.uint32, cap),
+                "slot_count": self._d2h(a.slot_count, np.uint32, cap),
+                "quads": self._d2h(a.quads, np.uint32, cap * q),
+                "load_xyz": self._d2h(a.load_xyz, np.int32, 3 * self.desc.max_loads),
+                "load_slot": self._d2h(a.load_slot, np.uint32, self.desc.max_loads)}
+
+    def destroy(self):
+        if self.h:
+            self.ctx.lib.meso_chunk_pool_destroy(self.h)
+            self.h = None

@@ -1,0 +1,110 @@
+"""Chunk streaming around a moving camera (meso_chunk_pool_*).
+
+Two measurements:
+  fly-through: the 2048^3 two-level cave volume; the camera moves one chunk
+    per update along +x while turning, and each update ranks the chunks of
+    its view sphere (radius 24 chunks forward, 6 backward, 120 degree cone),
+    loads at most 256 of them (the reference's default load limit) into a
+    16,384-slot pool (the reference's default MaxChunkCount), evicting the
+    least important, and greedy-meshes them into their slots.  The untimed
+    warm-up updates fill the pool, so the timed ones also evict.  Timed per
+    update with events; reported as ms per update and Mvoxels meshed per
+    second.  A second pool loads all 16,384 slots in one update (a cold
+    start).
+  sample: the 512^3 heightfield with a 4,096-slot pool and radius 16, two
+    updates; the same updates run on the CPU oracle in bench.py's CPU leg.
+"""
+import time
+
+from . import _lib as L
+from . import runtime as R
+from . import workloads as W
+
+FLY = dict(capacity=16384, quads_per_slot=8192, forward_radius=24, backward_radius=6,
+           view_angle=120.0, max_loads=256)
+SAMPLE = dict(capacity=4096, quads_per_slot=8192, forward_radius=16, backward_radius=6,
+              view_angle=120.0, max_loads=256)
+SAMPLE_PATH = [((8, 4, 8), (1.0, -0.2, 0.5)), ((9, 4, 8), (1.0, -0.2, 0.6))]
+
+
+def fly_path(k):
+    """camera chunk and forward vector of update k of the fly-through"""
+    import math
+    th = 0.05 * k
+    return (4 + k, 32, 30 + int(4 * math.sin(0.1 * k))), (math.cos(th), 0.15, math.sin(th))
+
+
+def _timed_update(ctx, pool, cam, fwd):
+    e0, e1 = R.Event(ctx), R.Event(ctx)
+    e0.record()
+    pool.update(cam, fwd)
+    e1.record()
+    e1.sync()
+    return e0.elapsed_ms(e1)
+
+
+def run(ctx, updates=24, warmup=72, log=print):
+    cfg = W.CAVES
+    t0 = time.time()
+    sc = ctx.scene_generate(L.GEN_CAVES, L.SCENE_TWO_LEVEL_U16, cfg["n"], cfg["seed"])
+    ctx.sync()
+    log("streaming: 2048^3 caves generated in %.2fs" % (time.time() - t0))
+    pool = R.ChunkPool(ctx, sc, mode=L.MESH_GREEDY, **FLY)
+    for k in range(warmup):
+        pool.update(*fly_path(k))
+    ctx.sync()
+    ms, loads, evicted, cands = [], 0, 0, []
+    for k in range(warmup, warmup + updates):
+        ms.append(_timed_update(ctx, pool, *fly_path(k)))
+        st = pool.stats()
+        loads += st["n_loads"]
+        evicted += st["n_evicted"]
+        cands.append(st["n_candidates"])
+    st = pool.stats()
+    med = sorted(ms)[len(ms) // 2]
+    total_ms = sum(ms)
+    fly = dict(updates=updates, ms_per_update_median=round(med, 4),
+               ms_per_update_mean=round(total_ms / updates, 4),
+               loads_per_update=round(loads / updates, 1),
+               evicted_per_update=round(evicted / updates, 1),
+               candidates_per_update=round(sum(cands) / updates, 1),
+               mvox_per_s=round(loads * 32768 / (total_ms * 1e-3) / 1e6, 1),
+               resident=st["n_resident"], resident_quads=st["n_quads"],
+               truncated=st["n_truncated"], pool_mib=round(st["device_bytes"] / 2 ** 20, 1),
+               params=FLY)
+    log("streaming: fly-through %.3f ms/update (median), %.0f loads, %.0f candidates, "
+        "%.0f Mvox/s" % (med, fly["loads_per_update"], fly["candidates_per_update"],
+                         fly["mvox_per_s"]))
+    pool.destroy()
+    # cold start: every slot in one update
+    cold = dict(FLY, max_loads=FLY["capacity"])
+    pool = R.ChunkPool(ctx, sc, mode=L.MESH_GREEDY, **cold)
+    pool.update(*fly_path(0))  # untimed first use
+    pool.destroy()
+    pool = R.ChunkPool(ctx, sc, mode=L.MESH_GREEDY, **cold)
+    ms0 = _timed_update(ctx, pool, *fly_path(0))
+    st = pool.stats()
+    fill = dict(ms=round(ms0, 3), loads=st["n_loads"], candidates=st["n_candidates"],
+                mvox_per_s=round(st["n_loads"] * 32768 / (ms0 * 1e-3) / 1e6, 1))
+    log("streaming: cold start %d loads in %.3f ms (%.0f Mvox/s)" % (
+        fill["loads"], ms0, fill["mvox_per_s"]))
+    pool.destroy()
+    sc.destroy()
+    # the CPU-comparable sample
+    t = W.TERRAIN
+    sc = ctx.scene_generate(L.GEN_HEIGHTFIELD, L.SCENE_BRICKMAP_U16, t["n"], t["seed"])
+    pool = R.ChunkPool(ctx, sc, mode=L.MESH_GREEDY, **SAMPLE)
+    pool.update(*SAMPLE_PATH[0])  # untimed first use
+    pool.destroy()
+    pool = R.ChunkPool(ctx, sc, mode=L.MESH_GREEDY, **SAMPLE)
+    sample_ms, sample_loads = 0.0, 0
+    for cam, fwd in SAMPLE_PATH:
+        sample_ms += _timed_update(ctx, pool, cam, fwd)
+        sample_loads += pool.stats()["n_loads"]
+    pool.destroy()
+    sc.destroy()
+    sample = dict(ms=round(sample_ms, 3), loads=sample_loads,
+                  mvox_per_s=round(sample_loads * 32768 / (sample_ms * 1e-3) / 1e6, 1),
+                  workload="512^3 heightfield brick map, %d slots, radius %d, %d updates" % (
+                      SAMPLE["capacity"], SAMPLE["forward_radius"], len(SAMPLE_PATH)))
+    return dict(fly_through=fly, cold_start=fill, sample=sample)

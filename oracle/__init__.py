@@ -59,6 +59,17 @@ class Shading(C.Structure):
                 ("ao_radius", C.c_float)]
 
 
+class PoolParams(C.Structure):
+    _fields_ = [("forward_radius", C.c_int32), ("backward_radius", C.c_int32),
+                ("view_threshold", C.c_float), ("capacity", C.c_int32),
+                ("quads_per_slot", C.c_int32), ("max_loads", C.c_int32), ("greedy", C.c_int32)]
+
+
+class PoolResult(C.Structure):
+    _fields_ = [("n_candidates", C.c_uint32), ("n_loads", C.c_uint32),
+                ("n_evicted", C.c_uint32)]
+
+
 class ChunkParams(C.Structure):
     _fields_ = [("fill", C.c_float), ("noise_seed", C.c_uint32), ("n_spheres", C.c_int32),
                 ("sph", (C.c_int32 * 4) * 4)]
@@ -109,6 +120,12 @@ def lib():
             "mo_build_coarse": (None, [P, C.c_int64, P, P, P]),
             "mo_edit_chunks": (None, [P, P, P, P, C.c_int32, C.c_uint32]),
             "mo_scene_chunk": (None, [P, C.c_int32, C.c_int32, C.c_int32, P]),
+            "mo_pool_view_threshold": (C.c_float, [C.c_float]),
+            "mo_pool_forward": (None, [P, P]),
+            "mo_chunk_importance": (C.c_float, [C.c_int32] * 3 + [P]),
+            "mo_chunk_desired": (C.c_int, [C.c_int32] * 3 + [P, P]),
+            "mo_chunk_nonempty": (C.c_int, [P] + [C.c_int32] * 3),
+            "mo_pool_update": (None, [P, P, C.c_uint32, P, P, P, P, P, P, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -411,3 +428,75 @@ def scene_chunk(scene, cx, cy, cz):
     out = np.zeros(32768, np.uint16)
     lib().mo_scene_chunk(scene.ptr, cx, cy, cz, _p(out))
     return out
+
+
+# ---------------- chunk pool ----------------
+NO_SLOT = 0xFFFFFFFF
+
+
+def pool_view_threshold(view_angle_deg):
+    return lib().mo_pool_view_threshold(view_angle_deg)
+
+
+def pool_forward(raw):
+    r = np.ascontiguousarray(raw, np.float32)
+    f = np.zeros(3, np.float32)
+    lib().mo_pool_forward(_p(r), _p(f))
+    return f
+
+
+def chunk_importance(dx, dy, dz, f):
+    f = np.ascontiguousarray(f, np.float32)
+    return lib().mo_chunk_importance(dx, dy, dz, _p(f))
+
+
+def pool_params(forward_radius=24, backward_radius=6, view_angle=120.0, capacity=16384,
+                quads_per_slot=8192, max_loads=256, greedy=True):
+    return PoolParams(forward_radius, backward_radius, pool_view_threshold(view_angle), capacity,
+                      quads_per_slot, max_loads, 1 if greedy else 0)
+
+
+def chunk_desired(dx, dy, dz, f, params):
+    f = np.ascontiguousarray(f, np.float32)
+    return bool(lib().mo_chunk_desired(dx, dy, dz, _p(f), C.byref(params)))
+
+
+def chunk_nonempty(scene, cx, cy, cz):
+    return bool(lib().mo_chunk_nonempty(scene.ptr, cx, cy, cz))
+
+
+class OraclePool:
+    """The chunk pool's state (lookup table, slots, quads) and its updates."""
+
+    def __init__(self, scene, params):
+        self.scene, self.p = scene, params
+        self.nc = -(-scene.n // 32)
+        self.lookup = np.full(self.nc ** 3, NO_SLOT, np.uint32)
+        cap = params.capacity
+        self.slot_chunk = np.full(cap, NO_SLOT, np.uint32)
+        self.slot_stamp = np.zeros(cap, np.uint32)
+        self.slot_count = np.zeros(cap, np.uint32)
+        self.quads = np.zeros(cap * params.quads_per_slot, np.uint32)
+        self.load_xyz = np.zeros(3 * max(params.max_loads, 1), np.int32)
+        self.load_slot = np.zeros(max(params.max_loads, 1), np.uint32)
+        self.stamp = 0
+
+    def update(self, cam_chunk, forward_raw):
+        """one update; returns (n_candidates, n_loads, n_evicted, loads [(x, y, z, slot)])"""
+        self.stamp = (self.stamp + 1) & 0xFFFFFFFF
+        f = pool_forward(forward_raw)
+        cam = np.ascontiguousarray(cam_chunk, np.int32)
+        r = PoolResult()
+        lib().mo_pool_update(self.scene.ptr, C.byref(self.p), self.stamp, _p(cam), _p(f),
+                             _p(self.lookup), _p(self.slot_chunk), _p(self.slot_stamp),
+                             _p(self.slot_count), _p(self.quads), _p(self.load_xyz),
+                             _p(self.load_slot), C.byref(r))
+        n = r.n_loads
+        loads = [tuple(int(v) for v in self.load_xyz[3 * i:3 * i + 3]) + (int(self.load_slot[i]),)
+                 for i in range(n)]
+        return r.n_candidates, n, r.n_evicted, loads
+
+    def slot_quads(self, slot):
+        q = self.p.quads_per_slot
+        k = min(int(self.slot_count[slot]), q)
+        return self.quads[slot * q: slot * q + k]

@@ -1214,3 +1214,164 @@ void mo_scene_chunk(const mo_scene *s, int32_t cx, int32_t cy, int32_t cz, uint1
                                   : s->voxels[(int64_t)slot * 512 + (gx & 7) + 8 * (gy & 7) + 64 * (gz & 7)];
       }
 }
+
+/* ------------------------------------------------------------------ */
+/* chunk pool                                                           */
+/* ------------------------------------------------------------------ */
+
+float mo_pool_view_threshold(float view_angle_deg) {
+  float r = view_angle_deg * 0.017453292f; /* degrees to radians */
+  r = r * 0.5f;
+  float c = cosf(r);
+  return c > 0.01f ? c : 0.01f;
+}
+
+void mo_pool_forward(const float raw[3], float f[3]) {
+  float ss = raw[0] * raw[0] + raw[1] * raw[1];
+  ss = ss + raw[2] * raw[2];
+  const float len = sqrtf(ss);
+  for (int i = 0; i < 3; i++) f[i] = raw[i] / len;
+}
+
+/* length of an integer offset and the dot of its direction with f */
+static float offset_dir_dot(int32_t dx, int32_t dy, int32_t dz, const float f[3], float *len) {
+  const float x = (float)dx, y = (float)dy, z = (float)dz;
+  float ss = x * x + y * y;
+  ss = ss + z * z;
+  *len = sqrtf(ss);
+  const float ux = x / *len, uy = y / *len, uz = z / *len;
+  float d = ux * f[0] + uy * f[1];
+  d = d + uz * f[2];
+  return d;
+}
+
+float mo_chunk_importance(int32_t dx, int32_t dy, int32_t dz, const float f[3]) {
+  if (dx >= -2 && dx <= 2 && dy >= -2 && dy <= 2 && dz >= -2 && dz <= 2) return 1.0e6f;
+  float len;
+  float a = offset_dir_dot(dx, dy, dz, f, &len);
+  a = a > 0.0f ? a : 0.0f;
+  a = a - 0.5f;
+  a = a * 2.0f;
+  a = a > 0.75f ? a : 0.75f;
+  float di = 64.0f - len;
+  di = di > 0.25f ? di : 0.25f;
+  return a * di;
+}
+
+int mo_chunk_desired(int32_t dx, int32_t dy, int32_t dz, const float f[3],
+                     const mo_pool_params *p) {
+  const float x = (float)dx, y = (float)dy, z = (float)dz;
+  float ss = x * x + y * y;
+  ss = ss + z * z;
+  const float len0 = sqrtf(ss);
+  if ((double)len0 > (double)p->forward_radius + 1e-6) return 0;
+  if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) return 1;
+  float len;
+  float a = offset_dir_dot(dx, dy, dz, f, &len);
+  a = a > 0.0f ? a : 0.0f;
+  a = a > p->view_threshold ? 1.0f : a / p->view_threshold;
+  a = a < 0.0f ? 0.0f : (a > 1.0f ? 1.0f : a);
+  const float t1 = a * (float)p->forward_radius;
+  const float b = 1.0f - a;
+  const float t2 = b * (float)p->backward_radius;
+  const float thr = t1 + t2;
+  return len < thr;
+}
+
+int mo_chunk_nonempty(const mo_scene *s, int32_t cx, int32_t cy, int32_t cz) {
+  const int32_t nc = (s->n + 31) / 32;
+  if (cx < 0 || cy < 0 || cz < 0 || cx >= nc || cy >= nc || cz >= nc) return 0;
+  if (s->kind == MO_SCENE_BRICKMAP_U16) return get_bit(s->occ32, lin3(cx, cy, cz, nc));
+  const uint32_t node = s->top[lin3(cx >> 2, cy >> 2, cz >> 2, s->n / 128)];
+  if (node == MO_EMPTY_SLOT) return 0;
+  return (int)((s->groups[node] >> ((cx & 3) + 4 * (cy & 3) + 16 * (cz & 3))) & 1u);
+}
+
+static uint32_t float_bits(float v) {
+  uint32_t u;
+  memcpy(&u, &v, 4);
+  return u;
+}
+
+static int cmp_u64(const void *a, const void *b) {
+  const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+void mo_pool_update(const mo_scene *s, const mo_pool_params *p, uint32_t stamp,
+                    const int32_t cam[3], const float f[3], uint32_t *lookup,
+                    uint32_t *slot_chunk, uint32_t *slot_stamp, uint32_t *slot_count,
+                    uint32_t *quads, int32_t *load_xyz, uint32_t *load_slot,
+                    mo_pool_result *res) {
+  const int32_t nc = (s->n + 31) / 32, R = p->forward_radius, side = 2 * R + 1;
+  const int64_t cube = (int64_t)side * side * side;
+  uint64_t *cand = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)cube);
+  uint64_t *vict = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)p->capacity);
+  uint32_t n_cand = 0;
+  for (int64_t k = 0; k < cube; k++) {
+    const int32_t dx = (int32_t)(k % side) - R, dy = (int32_t)((k / side) % side) - R,
+                  dz = (int32_t)(k / ((int64_t)side * side)) - R;
+    const int32_t cx = cam[0] + dx, cy = cam[1] + dy, cz = cam[2] + dz;
+    if (cx < 0 || cy < 0 || cz < 0 || cx >= nc || cy >= nc || cz >= nc) continue;
+    const uint32_t idx = (uint32_t)lin3(cx, cy, cz, nc);
+    if (lookup[idx] != MO_NO_SLOT || !mo_chunk_desired(dx, dy, dz, f, p) ||
+        !mo_chunk_nonempty(s, cx, cy, cz))
+      continue;
+    const float imp = mo_chunk_importance(dx, dy, dz, f);
+    cand[n_cand++] = ((uint64_t)(~float_bits(imp)) << 32) | idx;
+  }
+  qsort(cand, n_cand, sizeof(uint64_t), cmp_u64);
+  for (int32_t sl = 0; sl < p->capacity; sl++) {
+    const uint32_t c = slot_chunk[sl];
+    if (c == MO_NO_SLOT) {
+      vict[sl] = (uint64_t)sl;
+      continue;
+    }
+    const int32_t x = (int32_t)(c % (uint32_t)nc), y = (int32_t)((c / (uint32_t)nc) % (uint32_t)nc),
+                  z = (int32_t)(c / ((uint32_t)nc * (uint32_t)nc));
+    const float imp = mo_chunk_importance(x - cam[0], y - cam[1], z - cam[2], f);
+    vict[sl] = ((uint64_t)float_bits(imp) << 32) | (uint32_t)sl;
+  }
+  qsort(vict, (size_t)p->capacity, sizeof(uint64_t), cmp_u64);
+  uint32_t n = 0, evicted = 0;
+  while (n < n_cand && (int32_t)n < p->max_loads && (int32_t)n < p->capacity) {
+    const uint32_t vb = (uint32_t)(vict[n] >> 32);
+    float vi, ci;
+    const uint32_t cb = ~(uint32_t)(cand[n] >> 32);
+    memcpy(&vi, &vb, 4);
+    memcpy(&ci, &cb, 4);
+    if (vb != 0 && !(vi < ci)) break;
+    n++;
+  }
+  uint16_t *vox = (uint16_t *)malloc(32768 * sizeof(uint16_t));
+  uint32_t *tmp = (uint32_t *)malloc(98304 * sizeof(uint32_t));
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t sl = (uint32_t)vict[i], c = (uint32_t)cand[i];
+    const uint32_t old = slot_chunk[sl];
+    if (old != MO_NO_SLOT) {
+      lookup[old] = MO_NO_SLOT;
+      evicted++;
+    }
+    lookup[c] = sl;
+    slot_chunk[sl] = c;
+    slot_stamp[sl] = stamp;
+    const int32_t x = (int32_t)(c % (uint32_t)nc), y = (int32_t)((c / (uint32_t)nc) % (uint32_t)nc),
+                  z = (int32_t)(c / ((uint32_t)nc * (uint32_t)nc));
+    load_xyz[3 * i] = x;
+    load_xyz[3 * i + 1] = y;
+    load_xyz[3 * i + 2] = z;
+    load_slot[i] = sl;
+    mo_scene_chunk(s, x, y, z, vox);
+    const uint32_t cnt = mo_mesh_chunk(vox, 2, p->greedy, tmp, 98304);
+    const uint32_t keep = cnt < (uint32_t)p->quads_per_slot ? cnt : (uint32_t)p->quads_per_slot;
+    memcpy(quads + (uint64_t)sl * (uint32_t)p->quads_per_slot, tmp, keep * sizeof(uint32_t));
+    slot_count[sl] = cnt;
+  }
+  free(tmp);
+  free(vox);
+  free(cand);
+  free(vict);
+  res->n_candidates = n_cand;
+  res->n_loads = n;
+  res->n_evicted = evicted;
+}

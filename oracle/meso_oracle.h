@@ -34,6 +34,10 @@
  *   Shading: FP64 recomputation of every hit pixel's shade (lambert, shadow
  *     and AO occlusion by brute-force slab tests against every solid voxel).
  *   Edits / scene chunks: numpy restatement over the dense grid.
+ *   Chunk pool: closed-form importance values; the desired set against
+ *     integer lattice counts (equal radii) and cone cases; update invariants
+ *     (lookup and slots inverse maps, top-ranked loads, least important
+ *     evictions, load limit) against a Python ranking over float64.
  *   Meshers: brute-force face enumeration; closed forms (a x b x c box);
  *     greedy tiling invariants; erosion-cull cross-check.
  *   Reference TestGenerator (sin hash): parity unpinned (CPU timing only).
@@ -247,6 +251,57 @@ void mo_edit_chunks(const mo_scene *s, uint16_t *voxels, uint64_t *masks, const 
                     int32_t n_chunks, uint32_t seed);
 /* The 32^3 u16 voxels of chunk (cx, cy, cz) of a brick scene (x fastest). */
 void mo_scene_chunk(const mo_scene *s, int32_t cx, int32_t cy, int32_t cz, uint16_t *out);
+
+/* ---------------- chunk pool (streaming around a camera) ---------------- */
+/* A fixed number of slots holds the meshed quads of the 32^3 chunks around
+ * the camera, chosen by the reference's importance ranking
+ * (Runtimes/Voxel/Chunk/ChunkManagerHelper.h:21-43 importance,
+ * :91-148 desired set by view, ChunkPool.h:446-560 FindMin replacement,
+ * ChunkManager.h:211-260 the per-update load limit with the rest re-queued).
+ * Offsets (dx, dy, dz) are chunk offsets from the camera chunk; f is the unit
+ * forward vector (mo_pool_forward).  All float expressions are written one
+ * IEEE binary32 operation at a time; length = sqrt((x*x + y*y) + z*z) and
+ * normalize = v / length. */
+#define MO_NO_SLOT 0xFFFFFFFFu
+typedef struct {
+  int32_t forward_radius;  /* ViewForwardLoadChunkSize, chunks */
+  int32_t backward_radius; /* ViewBackwardLoadChunkSize */
+  float view_threshold;    /* max(cos(ViewChunkAngle / 2), 0.01) (mo_pool_view_threshold) */
+  int32_t capacity;        /* slots */
+  int32_t quads_per_slot;  /* quad capacity of a slot; longer meshes are cut (count kept) */
+  int32_t max_loads;       /* chunks loaded per update; the rest wait for later updates */
+  int32_t greedy;          /* 1 greedy, 0 culled */
+} mo_pool_params;
+typedef struct {
+  uint32_t n_candidates; /* desired, non-empty, not resident */
+  uint32_t n_loads;      /* loaded this update */
+  uint32_t n_evicted;    /* resident chunks replaced this update */
+} mo_pool_result;
+float mo_pool_view_threshold(float view_angle_deg);
+void mo_pool_forward(const float raw[3], float f[3]);
+/* FImportanceComputeInfo::CalculateChunkImportance (ChunkManagerHelper.h:21-43) */
+float mo_chunk_importance(int32_t dx, int32_t dy, int32_t dz, const float f[3]);
+/* GetDesiredShowChunkLocationByView's acceptance (ChunkManagerHelper.h:98-135) */
+int mo_chunk_desired(int32_t dx, int32_t dy, int32_t dz, const float f[3],
+                     const mo_pool_params *p);
+/* 1 when the chunk holds a non-empty brick (the scene's 32^3 level) */
+int mo_chunk_nonempty(const mo_scene *s, int32_t cx, int32_t cy, int32_t cz);
+/* One update.  Candidates: desired chunks inside the scene that are non-empty
+ * and not resident, ordered by importance (descending), then chunk index.
+ * Victims: the slots, free ones first (by slot), then resident chunks by
+ * importance under this camera (ascending), then slot.  Candidate i takes
+ * victim i while i < max_loads and the victim is free or strictly less
+ * important (FindMin: a resident chunk is replaced only by a more important
+ * one); the first failure ends the update (both orders are monotone).  Each
+ * loaded chunk is meshed from the scene into its slot (quads at
+ * slot * quads_per_slot, count in slot_count).  lookup: nc^3 entries (slot or
+ * MO_NO_SLOT), slot_chunk / slot_stamp / slot_count: capacity entries;
+ * load_xyz / load_slot: max_loads entries (the loads, in candidate order). */
+void mo_pool_update(const mo_scene *s, const mo_pool_params *p, uint32_t stamp,
+                    const int32_t cam[3], const float f[3], uint32_t *lookup,
+                    uint32_t *slot_chunk, uint32_t *slot_stamp, uint32_t *slot_count,
+                    uint32_t *quads, int32_t *load_xyz, uint32_t *load_slot,
+                    mo_pool_result *res);
 
 #ifdef __cplusplus
 }
