@@ -75,6 +75,7 @@ FHipVoxelInstance::FHipVoxelInstance(const FInstanceDesc &desc) : desc_(desc) {
 }
 
 FHipVoxelInstance::~FHipVoxelInstance() {
+  if (pool_) meso_chunk_pool_destroy(pool_);
   for (meso_event *e : ev_)
     if (e) meso_event_destroy(e);
   for (meso_buffer *b : {index_, face_, t_, shade_, dirty_, counts_, prefix_, quads_, total_, work_})
@@ -122,6 +123,20 @@ void FHipVoxelInstance::InitializeBegin() {
     return;
   for (meso_event *&e : ev_)
     if (!Check(meso_event_create(ctx_, &e))) return;
+  if (desc_.chunk_streaming) {
+    // the pool of ChunkPool.h:250-300, sized by the scene config
+    const FVoxelSceneConfig &c = VoxelSceneConfig;
+    meso_pool_desc pd{};
+    pd.size = sizeof(pd);
+    pd.capacity = (int32_t)c.MaxChunkCount;
+    pd.quads_per_slot = (int32_t)c.ChunkQuadCapacity;
+    pd.forward_radius = (int32_t)c.ViewForwardLoadChunkSize;
+    pd.backward_radius = (int32_t)c.ViewBackwardLoadChunkSize;
+    pd.view_angle = c.ViewChunkAngle;
+    pd.max_loads = (int32_t)(c.MaxSyncedLoadChunkCount + c.MaxUnsyncedLoadChunkCount);
+    pd.mesh_mode = desc_.mesh_mode;
+    Check(meso_chunk_pool_create(ctx_, scene_, &pd, &pool_));
+  }
 }
 
 void FHipVoxelInstance::QueueChunkEdits(const std::vector<int32_t> &xyz, uint32_t seed) {
@@ -138,6 +153,27 @@ void FHipVoxelInstance::WhenCameraUpdate() {
 void FHipVoxelInstance::UpdatePhysics() {
   last_ = FFrameStats{};
   physics_ran_ = false;
+  const int32_t n_dirty = (int32_t)(queued_xyz_.size() / 3);
+  ApplyEdits();
+  if (status_ != MESO_OK || !pool_) return;
+  // the loading queue (FChunkManage::UpdateLoadingQueue, ChunkManager.h:211-330):
+  // remesh the resident chunks the edits touched, then rank and load
+  const int32_t *d_xyz = (const int32_t *)meso_buffer_device_address(dirty_, 0);
+  const FIVec3 &cc = WindowsCamera.CameraChunkLocation;
+  const FVec3 f = WindowsCamera.GetForwardVector();
+  stream_chunk_[0] = cc.x;
+  stream_chunk_[1] = cc.y;
+  stream_chunk_[2] = cc.z;
+  stream_forward_[0] = f.x;
+  stream_forward_[1] = f.y;
+  stream_forward_[2] = f.z;
+  (void)(Check(meso_event_record(ev_[4], nullptr)) &&
+         (!physics_ran_ || Check(meso_chunk_pool_remesh(pool_, d_xyz, n_dirty, nullptr))) &&
+         Check(meso_chunk_pool_update(pool_, stream_chunk_, stream_forward_, nullptr)) &&
+         Check(meso_event_record(ev_[5], nullptr)));
+}
+
+void FHipVoxelInstance::ApplyEdits() {
   if (status_ != MESO_OK || queued_xyz_.empty()) return;
   const int32_t n = (int32_t)(queued_xyz_.size() / 3);
   if (n > desc_.max_dirty_chunks) {
@@ -198,6 +234,15 @@ void FHipVoxelInstance::RenderEnd() {
   float ms = 0.0f;
   if (!Check(meso_event_elapsed_ms(ev_[2], ev_[3], &ms))) return;
   last_.render_ms = ms;
+  if (pool_) {
+    if (!Check(meso_event_elapsed_ms(ev_[4], ev_[5], &ms))) return;
+    last_.stream_ms = ms;
+    meso_pool_stats ps{};
+    ps.size = sizeof(ps);
+    if (!Check(meso_chunk_pool_stats(pool_, &ps, nullptr))) return;
+    last_.pool_loads = ps.n_loads;
+    last_.pool_resident = ps.n_resident;
+  }
   if (physics_ran_) {
     if (!Check(meso_event_elapsed_ms(ev_[0], ev_[1], &ms))) return;
     last_.physics_ms = ms;

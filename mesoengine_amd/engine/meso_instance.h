@@ -32,13 +32,21 @@ struct FIVec3 {
 };
 
 // The scene geometry the path uses (Runtimes/Voxel/VoxelSceneConfig.h:20-49).
-// ChunkResolution is the edge, in voxels, of the chunks that edits, remeshing
-// and the camera's chunk tracking work on: 32 here (the reference's chunks
-// are 16 blocks of 8^3).
+// ChunkResolution is the edge, in voxels, of the chunks that edits, remeshing,
+// streaming and the camera's chunk tracking work on: 32 here (the reference's
+// chunks are 16 blocks of 8^3).  The streaming fields keep the reference's
+// names and defaults; ChunkQuadCapacity (quads a pool slot holds) is new.
 struct FVoxelSceneConfig {
   unsigned char BlockResolution = 8;
   float BlockSize = 1.0f;
   unsigned char ChunkResolution = 32;
+  uint32_t MaxChunkCount = 8192 * 2;
+  uint32_t ViewForwardLoadChunkSize = 24;
+  uint32_t ViewBackwardLoadChunkSize = 6;
+  uint32_t MaxSyncedLoadChunkCount = 0;
+  uint32_t MaxUnsyncedLoadChunkCount = 256;
+  float ViewChunkAngle = 120.0f;
+  uint32_t ChunkQuadCapacity = 8192;
   float GetChunkSize() const { return (float)ChunkResolution * BlockSize; }
 };
 
@@ -120,6 +128,9 @@ struct FFrameStats {
   float render_ms = 0.0f;
   uint64_t quads = 0;       // quads of the chunks remeshed this frame
   int32_t dirty_chunks = 0;
+  float stream_ms = 0.0f;   // chunk streaming: remesh of resident dirty chunks + pool update
+  uint32_t pool_loads = 0;  // chunks the pool loaded this frame
+  uint32_t pool_resident = 0;
 };
 
 struct FInstanceDesc {
@@ -137,6 +148,9 @@ struct FInstanceDesc {
   uint32_t mesh_mode = MESO_MESH_GREEDY;
   int32_t max_dirty_chunks = 4096;  // per frame
   uint64_t quad_capacity = 16u << 20;
+  // keep the chunks around the camera meshed in a meso_chunk_pool sized by
+  // VoxelSceneConfig (the reference's ChunkManager streaming); brick scenes
+  bool chunk_streaming = false;
 };
 
 class FHipVoxelInstance {
@@ -161,6 +175,8 @@ class FHipVoxelInstance {
   virtual void UpdateFrameIndex();
   virtual void WhenCameraChunkUpdate() {}
   virtual void WhenCameraUpdate();
+  // the queued chunk edits and the remesh of the dirty chunks (part of UpdatePhysics)
+  void ApplyEdits();
 
   // chunk edits applied by the next UpdatePhysics: chunk coordinates
   // (x, y, z triples) and the seed of their sphere toggles
@@ -173,6 +189,10 @@ class FHipVoxelInstance {
   uint32_t RenderGlobalFrameIndex() const { return frame_index_; }
   meso_context *Context() const { return ctx_; }
   meso_scene *Scene() const { return scene_; }
+  meso_chunk_pool *ChunkPool() const { return pool_; }
+  // the camera chunk and forward vector the last pool update used
+  const int32_t *StreamChunk() const { return stream_chunk_; }
+  const float *StreamForward() const { return stream_forward_; }
 
   // copies of the last frame's outputs and the last remesh (host memory)
   meso_status DownloadFrame(std::vector<uint8_t> &index, std::vector<uint8_t> &face,
@@ -191,7 +211,10 @@ class FHipVoxelInstance {
   meso_buffer *index_ = nullptr, *face_ = nullptr, *t_ = nullptr, *shade_ = nullptr;
   meso_buffer *dirty_ = nullptr, *counts_ = nullptr, *prefix_ = nullptr, *quads_ = nullptr;
   meso_buffer *total_ = nullptr, *work_ = nullptr;
-  meso_event *ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+  meso_event *ev_[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  meso_chunk_pool *pool_ = nullptr;
+  int32_t stream_chunk_[3] = {0, 0, 0};
+  float stream_forward_[3] = {1.0f, 0.0f, 0.0f};
   std::vector<int32_t> queued_xyz_;
   uint32_t queued_seed_ = 0;
   int32_t meshed_chunks_ = 0;

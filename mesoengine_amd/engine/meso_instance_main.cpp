@@ -159,15 +159,62 @@ int usage() {
                "  [--seed S] [--width W] [--height H] [--frames F] [--dirty D] [--ao K]\n"
                "  [--no-shadow] [--eye x,y,z] [--forward x,y,z] [--up x,y,z] [--step V]\n"
                "  [--mesh culled|greedy] [--dump FILE] [--device I]\n"
-               "  [--upload-bricks N --upload-batches B]\n");
+               "  [--upload-bricks N --upload-batches B]\n"
+               "  [--stream [--pool-capacity C --pool-radius R --pool-loads L] [--dump-pool FILE]]\n");
   return 2;
 }
 
 }  // namespace
 
+// The pool's state after the last frame and the per-frame inputs of its
+// updates, for a parity replay: "MESOPOOL", version, frames, capacity,
+// quads_per_slot, nc, then per frame the camera chunk (3 x i32) and forward
+// (3 x f32), then lookup (nc^3 u32), slot_chunk, slot_stamp, slot_count
+// (capacity u32 each) and every slot's quads (capacity x quads_per_slot u32).
+static bool write_pool_dump(const FHipVoxelInstance &inst, const std::string &path,
+                            const std::vector<int32_t> &chunks, const std::vector<float> &fwd) {
+  meso_pool_arrays a{};
+  a.size = sizeof(a);
+  if (meso_chunk_pool_arrays(inst.ChunkPool(), &a) != MESO_OK) return false;
+  const FVoxelSceneConfig &c = inst.VoxelSceneConfig;
+  const uint64_t cap = c.MaxChunkCount, q = c.ChunkQuadCapacity;
+  const uint64_t nc3 = (uint64_t)a.chunks_per_axis * a.chunks_per_axis * a.chunks_per_axis;
+  std::vector<uint32_t> lookup(nc3), sc(cap), ss(cap), cnt(cap), quads(cap * q);
+  meso_context *ctx = inst.Context();
+  if (meso_memcpy_d2h(ctx, lookup.data(), a.lookup, nc3 * 4, nullptr) != MESO_OK ||
+      meso_memcpy_d2h(ctx, sc.data(), a.slot_chunk, cap * 4, nullptr) != MESO_OK ||
+      meso_memcpy_d2h(ctx, ss.data(), a.slot_stamp, cap * 4, nullptr) != MESO_OK ||
+      meso_memcpy_d2h(ctx, cnt.data(), a.slot_count, cap * 4, nullptr) != MESO_OK ||
+      meso_memcpy_d2h(ctx, quads.data(), a.quads, cap * q * 4, nullptr) != MESO_OK) {
+    std::fprintf(stderr, "meso_instance: pool download: %s\n", meso_last_error_message());
+    return false;
+  }
+  FILE *fo = std::fopen(path.c_str(), "wb");
+  if (!fo) {
+    std::perror(path.c_str());
+    return false;
+  }
+  const uint32_t hdr[5] = {1u, (uint32_t)(chunks.size() / 3), (uint32_t)cap, (uint32_t)q,
+                           (uint32_t)a.chunks_per_axis};
+  std::fwrite("MESOPOOL", 1, 8, fo);
+  std::fwrite(hdr, 4, 5, fo);
+  for (size_t f = 0; f < chunks.size() / 3; f++) {
+    std::fwrite(&chunks[3 * f], 4, 3, fo);
+    std::fwrite(&fwd[3 * f], 4, 3, fo);
+  }
+  std::fwrite(lookup.data(), 4, nc3, fo);
+  std::fwrite(sc.data(), 4, cap, fo);
+  std::fwrite(ss.data(), 4, cap, fo);
+  std::fwrite(cnt.data(), 4, cap, fo);
+  std::fwrite(quads.data(), 4, cap * q, fo);
+  std::fclose(fo);
+  return true;
+}
+
 int main(int argc, char **argv) {
   FInstanceDesc d;
-  std::string scene = "caves", dump;
+  std::string scene = "caves", dump, dump_pool;
+  int pool_capacity = 0, pool_radius = 0, pool_loads = 0;
   uint32_t frames = 4;
   int32_t dirty = 64, device = 0;
   float eye[3] = {-1, -1, -1}, fwd[3] = {1.0f, -0.3f, 0.6f}, up[3] = {0.0f, 1.0f, 0.0f};
@@ -178,6 +225,10 @@ int main(int argc, char **argv) {
     const std::string a = argv[i];
     const char *v = i + 1 < argc ? argv[i + 1] : nullptr;
     auto need = [&]() { return v != nullptr; };
+    if (a == "--stream") {
+      d.chunk_streaming = true;
+      continue;
+    }
     if (a == "--no-shadow") {
       shadow = false;
       continue;
@@ -195,6 +246,10 @@ int main(int argc, char **argv) {
     else if (a == "--step") step = (float)std::atof(v);
     else if (a == "--device") device = std::atoi(v);
     else if (a == "--dump") dump = v;
+    else if (a == "--dump-pool") dump_pool = v;
+    else if (a == "--pool-capacity") pool_capacity = std::atoi(v);
+    else if (a == "--pool-radius") pool_radius = std::atoi(v);
+    else if (a == "--pool-loads") pool_loads = std::atoi(v);
     else if (a == "--upload-bricks") upload_bricks = std::atoi(v);
     else if (a == "--upload-batches") upload_batches = std::atoi(v);
     else if (a == "--mesh") d.mesh_mode = std::string(v) == "culled" ? MESO_MESH_CULLED : MESO_MESH_GREEDY;
@@ -225,6 +280,11 @@ int main(int argc, char **argv) {
 
   FHipVoxelInstance inst(d);
   inst.VoxelSceneConfig.ChunkResolution = 32;
+  if (pool_capacity > 0) inst.VoxelSceneConfig.MaxChunkCount = (uint32_t)pool_capacity;
+  if (pool_radius > 0) inst.VoxelSceneConfig.ViewForwardLoadChunkSize = (uint32_t)pool_radius;
+  if (pool_loads > 0) inst.VoxelSceneConfig.MaxUnsyncedLoadChunkCount = (uint32_t)pool_loads;
+  if (inst.VoxelSceneConfig.ViewBackwardLoadChunkSize > inst.VoxelSceneConfig.ViewForwardLoadChunkSize)
+    inst.VoxelSceneConfig.ViewBackwardLoadChunkSize = inst.VoxelSceneConfig.ViewForwardLoadChunkSize;
   if (inst.Initialize(device) != MESO_OK) {
     std::fprintf(stderr, "meso_instance: %s\n", inst.ErrorMessage().c_str());
     return 1;
@@ -245,6 +305,8 @@ int main(int argc, char **argv) {
   std::vector<uint32_t> seeds;
   std::vector<std::vector<int32_t>> sets;
   std::vector<FFrameStats> stats;
+  std::vector<int32_t> stream_chunks;  // per frame: the pool update's camera chunk
+  std::vector<float> stream_fwd;       // and forward vector
   for (uint32_t f = 0; f < frames; f++) {
     if (dirty > 0) {
       sets.push_back(dirty_set(nc, dirty, d.seed, f));
@@ -256,6 +318,8 @@ int main(int argc, char **argv) {
       return 1;
     }
     stats.push_back(inst.LastFrame());
+    stream_chunks.insert(stream_chunks.end(), inst.StreamChunk(), inst.StreamChunk() + 3);
+    stream_fwd.insert(stream_fwd.end(), inst.StreamForward(), inst.StreamForward() + 3);
     // fly along the view direction; the next UpdateCamera moves the origin
     inst.WindowsCamera.Position.x += step * fwd[0] / fl;
     inst.WindowsCamera.Position.y += step * fwd[1] / fl;
@@ -271,6 +335,15 @@ int main(int argc, char **argv) {
   std::printf("], \"quads\": [");
   for (size_t i = 0; i < stats.size(); i++)
     std::printf("%s%" PRIu64, i ? ", " : "", stats[i].quads);
+  if (d.chunk_streaming) {
+    std::printf("], \"stream_ms\": [");
+    for (size_t i = 0; i < stats.size(); i++) std::printf("%s%.4f", i ? ", " : "", stats[i].stream_ms);
+    std::printf("], \"pool_loads\": [");
+    for (size_t i = 0; i < stats.size(); i++) std::printf("%s%u", i ? ", " : "", stats[i].pool_loads);
+    std::printf("], \"pool_resident\": [");
+    for (size_t i = 0; i < stats.size(); i++)
+      std::printf("%s%u", i ? ", " : "", stats[i].pool_resident);
+  }
   const meso_camera &c = inst.FrameCamera();
   std::printf("], \"eye\": [%.9g, %.9g, %.9g]", c.eye[0], c.eye[1], c.eye[2]);
   if (upload_bricks > 0 && upload_batches > 0 && dirty > 0) {
@@ -284,6 +357,9 @@ int main(int argc, char **argv) {
   std::printf("}\n");
   std::fflush(stdout);
 
+  if (!dump_pool.empty() && inst.ChunkPool()) {
+    if (!write_pool_dump(inst, dump_pool, stream_chunks, stream_fwd)) return 1;
+  }
   if (!dump.empty()) {
     std::vector<uint8_t> index, face;
     std::vector<float> t, shade;

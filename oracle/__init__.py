@@ -496,6 +496,20 @@ class OraclePool:
                  for i in range(n)]
         return r.n_candidates, n, r.n_evicted, loads
 
+    def remesh(self, xyz):
+        """re-meshes the resident chunks of a dirty list into their slots"""
+        q, nc = self.p.quads_per_slot, self.nc
+        for x, y, z in np.asarray(xyz, np.int64).reshape(-1, 3):
+            if min(x, y, z) < 0 or max(x, y, z) >= nc:
+                continue
+            sl = int(self.lookup[x + nc * (y + nc * z)])
+            if sl == NO_SLOT:
+                continue
+            ref = mesh_chunk(scene_chunk(self.scene, int(x), int(y), int(z)), bool(self.p.greedy))
+            self.slot_count[sl] = len(ref)
+            k = min(len(ref), q)
+            self.quads[sl * q: sl * q + k] = ref[:k]
+
     def slot_quads(self, slot):
         q = self.p.quads_per_slot
         k = min(int(self.slot_count[slot]), q)

@@ -126,3 +126,69 @@ def test_double_buffered_brick_upload():
     assert p.returncode == 0, p.stderr
     up = json.loads(p.stdout.strip().splitlines()[-1])["upload"]
     assert up["check"] is True and up["batches"] == 5 and up["touched"] > 3000
+
+
+def _read_pool_dump(path):
+    b = open(path, "rb").read()
+    assert b[:8] == b"MESOPOOL"
+    version, frames, cap, q, nc = (int(v) for v in np.frombuffer(b, np.uint32, 5, 8))
+    assert version == 1
+    off = 28
+    chunks, fwd = [], []
+    for _ in range(frames):
+        chunks.append(tuple(int(v) for v in np.frombuffer(b, np.int32, 3, off)))
+        fwd.append(np.frombuffer(b, np.float32, 3, off + 12).copy())
+        off += 24
+    out = dict(frames=frames, cap=cap, q=q, nc=nc, chunks=chunks, fwd=fwd)
+    for name, count in (("lookup", nc ** 3), ("slot_chunk", cap), ("slot_stamp", cap),
+                        ("slot_count", cap), ("quads", cap * q)):
+        out[name] = np.frombuffer(b, np.uint32, count, off)
+        off += 4 * count
+    assert off == len(b)
+    return out
+
+
+def test_instance_chunk_streaming_matches_oracle(tmp_path):
+    """UpdatePhysics with --stream: edits, the remesh of resident dirty chunks
+    and the pool update every frame; the oracle pool replays the same inputs
+    and must end in the same state."""
+    assert os.path.exists(BIN), "engine driver not built"
+    n, seed, frames = 256, 5, 6
+    dump, pdump = str(tmp_path / "inst.bin"), str(tmp_path / "pool.bin")
+    cmd = [BIN, "--scene", "caves", "--n", str(n), "--seed", str(seed), "--width", "64",
+           "--height", "48", "--frames", str(frames), "--dirty", "24", "--ao", "0", "--step",
+           "24", "--stream", "--pool-capacity", "64", "--pool-radius", "4", "--pool-loads", "16",
+           "--dump", dump, "--dump-pool", pdump]
+    print("running", " ".join(cmd), flush=True)
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    rep = json.loads(p.stdout.strip().splitlines()[-1])
+    assert len(rep["stream_ms"]) == frames and all(ms > 0 for ms in rep["stream_ms"])
+    assert sum(rep["pool_loads"]) > 64, rep["pool_loads"]
+    d = _read_dump(dump)
+    pd = _read_pool_dump(pdump)
+    assert pd["frames"] == frames and len(set(pd["chunks"])) > 1, pd["chunks"]
+
+    slots, masks, vox = O.build_brickmap(O.GEN_CAVES, n, seed)
+    top, slots, masks, vox = O.two_level_from_bricks(n, slots, masks, vox)
+    masks, vox = np.ascontiguousarray(masks.copy()), np.ascontiguousarray(vox.copy())
+    osc = O.two_level_scene(n, top, slots, masks, vox)
+    pool = O.OraclePool(osc, O.pool_params(forward_radius=4, backward_radius=4, view_angle=120.0,
+                                           capacity=64, quads_per_slot=pd["q"], max_loads=16,
+                                           greedy=True))
+    loads = []
+    for f in range(frames):
+        s, xyz = d["sets"][f]
+        O.edit_chunks(osc, np.ascontiguousarray(xyz), s)
+        pool.remesh(xyz)
+        loads.append(pool.update(pd["chunks"][f], pd["fwd"][f])[1])
+    assert loads == rep["pool_loads"]
+    assert np.array_equal(pd["lookup"], pool.lookup)
+    assert np.array_equal(pd["slot_chunk"], pool.slot_chunk)
+    assert np.array_equal(pd["slot_stamp"], pool.slot_stamp)
+    res = pool.slot_chunk != O.NO_SLOT
+    assert np.array_equal(pd["slot_count"][res], pool.slot_count[res])
+    q = pd["q"]
+    for sl in np.nonzero(res)[0]:
+        k = min(int(pool.slot_count[sl]), q)
+        assert np.array_equal(pd["quads"][sl * q: sl * q + k], pool.quads[sl * q: sl * q + k]), sl
