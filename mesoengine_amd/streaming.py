@@ -1,8 +1,9 @@
 """Chunk streaming around a moving camera (meso_chunk_pool_*).
 
 Two measurements:
-  fly-through: the 2048^3 two-level cave volume; the camera moves one chunk
-    per update along +x while turning, and each update ranks the chunks of
+  fly-through: the 2048^3 two-level cave volume; the camera circles the
+    middle of the volume about one chunk per update, and each update ranks
+    the chunks of
     its view sphere (radius 24 chunks forward, 6 backward, 120 degree cone),
     loads at most 256 of them (the reference's default load limit) into a
     16,384-slot pool (the reference's default MaxChunkCount), evicting the
@@ -28,10 +29,13 @@ SAMPLE_PATH = [((8, 4, 8), (1.0, -0.2, 0.5)), ((9, 4, 8), (1.0, -0.2, 0.6))]
 
 
 def fly_path(k):
-    """camera chunk and forward vector of update k of the fly-through"""
+    """camera chunk and forward vector of update k of the fly-through: a
+    circle of radius 20 chunks around the middle of the 64^3-chunk volume,
+    about one chunk per update, looking along the circle"""
     import math
-    th = 0.05 * k
-    return (4 + k, 32, 30 + int(4 * math.sin(0.1 * k))), (math.cos(th), 0.15, math.sin(th))
+    th = k / 20.0
+    cam = (32 + int(round(20 * math.cos(th))), 32, 32 + int(round(20 * math.sin(th))))
+    return cam, (-math.sin(th), 0.15, math.cos(th))
 
 
 def _timed_update(ctx, pool, cam, fwd):
