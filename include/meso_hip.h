@@ -454,6 +454,43 @@ meso_status meso_chunk_pool_stats(meso_chunk_pool *pool, meso_pool_stats *out, m
 meso_status meso_chunk_pool_arrays(meso_chunk_pool *pool, meso_pool_arrays *out);
 
 /* ------------------------------------------------------------------ */
+/* quad visibility: the raster consumer of the meshes                    */
+/* ------------------------------------------------------------------ */
+/* Replaces the per-cell imposter draw of Samples/SimpleVoxel.cpp:352-398
+ * (a 3-face fan per solid cell, FTriplePlanarCube, Shape/TriplePlanarCube.h:31-46)
+ * with the packed quads of a chunk pool.  For every pixel, the nearest
+ * front-facing quad along the pixel's primary ray (the ray meso_raymarch
+ * casts): the ray must travel against the face normal, t = (plane - o) *
+ * (1 / d) > 0, and the hit point must lie in the quad's [u0, u0 + w) x
+ * [v0, v0 + h).  Ties go to the smaller slot * quads_per_slot + k.  The
+ * outputs are meso_raymarch's with flags 0: the voxel behind the face
+ * (-1 on a miss), the face, t (+inf on a miss) and the shade
+ * 0.25 + 0.75 * lambert (0 on a miss; shading may be NULL for light +y);
+ * counters are not written.  Where the pool holds every non-empty chunk the
+ * view reaches, the result equals the raymarcher's primary hit (t to the
+ * bit) except on rays through voxel edges. */
+typedef struct meso_raster meso_raster;
+typedef struct meso_raster_stats {
+  uint32_t size;           /* sizeof(meso_raster_stats) */
+  uint32_t small_quads;    /* last frame: quads of at most 64 pixels, tested by one lane */
+  uint32_t large_quads;    /* last frame: larger quads, cut into segments of 256 8x8 tiles */
+  uint32_t segments;       /* last frame: their segments */
+  uint32_t overflow_quads; /* last frame: large quads tested by one lane (segment list full) */
+} meso_raster_stats;
+/* Scratch for frames up to max_width x max_height (8 B per pixel plus a
+ * 32 MiB segment list). */
+meso_status meso_raster_create(meso_context *ctx, int32_t max_width, int32_t max_height,
+                               meso_raster **out);
+meso_status meso_raster_destroy(meso_raster *raster);
+/* One frame of the pool's resident quads; runs on `stream` without a host
+ * wait.  Calls on one raster object must be ordered. */
+meso_status meso_raster_pool(meso_raster *raster, meso_chunk_pool *pool, const meso_camera *cam,
+                             const meso_shading *shading, const meso_ray_outputs *out,
+                             meso_stream stream);
+/* Counts of the last frame; waits for `stream`. */
+meso_status meso_raster_stats_get(meso_raster *raster, meso_raster_stats *out, meso_stream stream);
+
+/* ------------------------------------------------------------------ */
 /* serving: continuous batching of raymarch requests over resident       */
 /* scenes, admitted by an HBM budget (the GPU counterpart of the         */
 /* streaming loop FChunkManage::UpdateLoadingQueue, ChunkManager.h:211-400, */

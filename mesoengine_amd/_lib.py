@@ -110,6 +110,11 @@ class PoolArrays(C.Structure):
                 ("load_slot", C.c_void_p)]
 
 
+class RasterStats(C.Structure):
+    _fields_ = [("size", C.c_uint32), ("small_quads", C.c_uint32), ("large_quads", C.c_uint32),
+                ("segments", C.c_uint32), ("overflow_quads", C.c_uint32)]
+
+
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
 _SIGS = {
@@ -181,6 +186,10 @@ _SIGS = {
     "meso_chunk_pool_remesh": (C.c_int32, [P, P, C.c_int32, P]),
     "meso_chunk_pool_stats": (C.c_int32, [P, P, P]),
     "meso_chunk_pool_arrays": (C.c_int32, [P, P]),
+    "meso_raster_create": (C.c_int32, [P, C.c_int32, C.c_int32, PP]),
+    "meso_raster_destroy": (C.c_int32, [P]),
+    "meso_raster_pool": (C.c_int32, [P, P, P, P, P, P]),
+    "meso_raster_stats_get": (C.c_int32, [P, P, P]),
 }
 EXPORTED = sorted(_SIGS)
 

@@ -10,7 +10,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libmeso_hip.so")
 SOURCES = ["api.hip", "raymarch.hip", "mesh.hip", "voxelize.hip", "serve.hip", "tune.hip",
-           "pool.hip"]
+           "pool.hip", "raster.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: no FMA contraction, every float op rounds as written.

@@ -227,7 +227,7 @@ void derive_shading(const meso_shading *s, DevShading *o) {
   o->ao_radius = s->ao_radius;
 }
 
-static meso_status check_camera(const meso_camera *cam) {
+meso_status check_camera(const meso_camera *cam) {
   MESO_CHECK(cam, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "camera is NULL");
   MESO_CHECK(cam->size >= sizeof(meso_camera), MESO_ERR_ARGUMENT_OUT_OF_RANGE,
              "meso_camera.size %u < %zu", cam->size, sizeof(meso_camera));

@@ -152,6 +152,15 @@ __device__ __forceinline__ uint64_t block_id() {
 // host helpers shared across translation units
 void derive_camera(const meso_camera *c, DevCamera *out);
 void derive_shading(const meso_shading *s, DevShading *out);
+// validates a caller's camera (size, image, fov, finite vectors, up not parallel)
+meso_status check_camera(const meso_camera *cam);
+// what the quad rasteriser reads of a chunk pool
+struct PoolView {
+  const meso_scene *scene;
+  const uint32_t *slot_chunk, *slot_count, *quads;
+  int32_t capacity, per_slot, nc;
+};
+PoolView pool_view(const meso_chunk_pool *pool);
 hipStream_t stream_of(meso_context *ctx, meso_stream s);
 meso_status launch_zero(void *ptr, uint64_t bytes, hipStream_t stream);
 // the persistent raymarch's band counters for `stream` (1 KiB, zeroed once)

@@ -306,6 +306,13 @@ static meso_status pool_alloc(meso_chunk_pool *p, void **ptr, uint64_t bytes) {
   return MESO_OK;
 }
 
+namespace meso {
+PoolView pool_view(const meso_chunk_pool *p) {
+  return PoolView{p->scene, p->slot_chunk, p->slot_count, p->quads, p->geom.capacity,
+                  p->desc.quads_per_slot, p->geom.nc};
+}
+}  // namespace meso
+
 extern "C" {
 
 meso_status meso_chunk_pool_create(meso_context *ctx, meso_scene *scene, const meso_pool_desc *desc,

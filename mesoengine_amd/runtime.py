@@ -406,3 +406,28 @@ class ChunkPool:
         if self.h:
             self.ctx.lib.meso_chunk_pool_destroy(self.h)
             self.h = None
+
+
+class Raster:
+    """Quad visibility of a chunk pool's resident quads (meso_raster_*)."""
+
+    def __init__(self, ctx, max_width, max_height):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(ctx.lib.meso_raster_create(ctx.h, max_width, max_height, C.byref(h)))
+        self.h = h
+
+    def draw(self, pool, cam, sh, targets, stream=None):
+        check(self.ctx.lib.meso_raster_pool(self.h, pool.h, byref(cam), byref(sh) if sh else None,
+                                            byref(targets.out), stream))
+
+    def stats(self, stream=None):
+        s = L.RasterStats()
+        s.size = C.sizeof(s)
+        check(self.ctx.lib.meso_raster_stats_get(self.h, byref(s), stream))
+        return {k: getattr(s, k) for k, _ in L.RasterStats._fields_ if k != "size"}
+
+    def destroy(self):
+        if self.h:
+            self.ctx.lib.meso_raster_destroy(self.h)
+            self.h = None

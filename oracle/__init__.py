@@ -126,6 +126,8 @@ def lib():
             "mo_chunk_desired": (C.c_int, [C.c_int32] * 3 + [P, P]),
             "mo_chunk_nonempty": (C.c_int, [P] + [C.c_int32] * 3),
             "mo_pool_update": (None, [P, P, C.c_uint32, P, P, P, P, P, P, P, P, P, P]),
+            "mo_raster_pool": (None, [P, C.c_int32, C.c_int32, P, P, P, C.c_int32, C.c_int32,
+                                      P, P, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -514,3 +516,21 @@ class OraclePool:
         q = self.p.quads_per_slot
         k = min(int(self.slot_count[slot]), q)
         return self.quads[slot * q: slot * q + k]
+
+
+# ---------------- quad visibility ----------------
+def raster_pool(cam, n, nc, slot_chunk, slot_count, quads, per_slot, light):
+    """(index, face, t, shade, keys) of the nearest front-facing quad per pixel"""
+    w, h = cam.width, cam.height
+    keys = np.empty(w * h, np.uint64)
+    idx = np.empty(w * h, np.int64)
+    face = np.empty(w * h, np.uint8)
+    t = np.empty(w * h, np.float32)
+    shade = np.empty(w * h, np.float32)
+    sc = np.ascontiguousarray(slot_chunk, np.uint32)
+    cnt = np.ascontiguousarray(slot_count, np.uint32)
+    q = np.ascontiguousarray(quads, np.uint32)
+    li = np.ascontiguousarray(light, np.float32)
+    lib().mo_raster_pool(C.byref(cam), n, nc, _p(sc), _p(cnt), _p(q), len(sc), per_slot, _p(li),
+                         _p(keys), _p(idx), _p(face), _p(t), _p(shade))
+    return idx, face, t, shade, keys

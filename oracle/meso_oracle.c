@@ -1375,3 +1375,169 @@ void mo_pool_update(const mo_scene *s, const mo_pool_params *p, uint32_t stamp,
   res->n_loads = n;
   res->n_evicted = evicted;
 }
+
+/* ------------------------------------------------------------------ */
+/* quad visibility                                                      */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  int a, u, v;       /* plane axis and the two in-plane axes */
+  int pos;           /* 1: the face normal points along +a */
+  float P;           /* plane coordinate */
+  int32_t U0, V0, W, H;
+  int32_t slice;     /* voxel coordinate along a of the face's voxel */
+} quad_geom;
+
+static void quad_decode(uint32_t q, int32_t ox, int32_t oy, int32_t oz, quad_geom *g) {
+  const int32_t x = (int32_t)(q & 31u), y = (int32_t)((q >> 5) & 31u), z = (int32_t)((q >> 10) & 31u);
+  const int32_t w = (int32_t)((q >> 15) & 31u) + 1, h = (int32_t)((q >> 20) & 31u) + 1;
+  const int f = (int)((q >> 25) & 7u);
+  const int32_t X = ox + x, Y = oy + y, Z = oz + z;
+  g->pos = (f & 1) == 0;
+  if (f < 2) {
+    g->a = 2; g->u = 0; g->v = 1; g->slice = Z; g->U0 = X; g->V0 = Y;
+  } else if (f < 4) {
+    g->a = 1; g->u = 0; g->v = 2; g->slice = Y; g->U0 = X; g->V0 = Z;
+  } else {
+    g->a = 0; g->u = 1; g->v = 2; g->slice = X; g->U0 = Y; g->V0 = Z;
+  }
+  g->W = w;
+  g->H = h;
+  g->P = (float)(g->slice + (g->pos ? 1 : 0));
+}
+
+/* t of the ray's hit on the quad, or -1 */
+static float quad_hit(const quad_geom *g, const float o[3], const float d[3]) {
+  const float da = d[g->a];
+  if (g->pos ? !(da < 0.0f) : !(da > 0.0f)) return -1.0f;
+  const float inv = 1.0f / da;
+  const float q = g->P - o[g->a];
+  const float t = q * inv;
+  if (!(t > 0.0f)) return -1.0f;
+  const float du = d[g->u] * t, dv = d[g->v] * t;
+  const float hu = o[g->u] + du, hv = o[g->v] + dv;
+  if (!(hu >= (float)g->U0 && hu < (float)(g->U0 + g->W))) return -1.0f;
+  if (!(hv >= (float)g->V0 && hv < (float)(g->V0 + g->H))) return -1.0f;
+  return t;
+}
+
+/* conservative pixel rectangle [x0, x1) x [y0, y1) of a quad; 0 if empty */
+static int quad_rect(const mo_camera *c, const quad_geom *g, int32_t r[4]) {
+  double cam[4][3], poly[8][3];
+  for (int k = 0; k < 4; k++) {
+    double p[3];
+    p[g->a] = g->P;
+    p[g->u] = g->U0 + ((k == 1 || k == 2) ? g->W : 0);
+    p[g->v] = g->V0 + ((k >= 2) ? g->H : 0);
+    double e[3] = {p[0] - c->eye[0], p[1] - c->eye[1], p[2] - c->eye[2]};
+    cam[k][0] = e[0] * c->right[0] + e[1] * c->right[1] + e[2] * c->right[2];
+    cam[k][1] = e[0] * c->up[0] + e[1] * c->up[1] + e[2] * c->up[2];
+    cam[k][2] = e[0] * c->fwd[0] + e[1] * c->fwd[1] + e[2] * c->fwd[2];
+  }
+  const double zn = 1e-3;
+  int m = 0;
+  for (int k = 0; k < 4; k++) { /* clip the polygon at z >= zn */
+    const double *A = cam[k], *B = cam[(k + 1) & 3];
+    const int ia = A[2] >= zn, ib = B[2] >= zn;
+    if (ia) {
+      poly[m][0] = A[0]; poly[m][1] = A[1]; poly[m][2] = A[2]; m++;
+    }
+    if (ia != ib) {
+      const double s = (zn - A[2]) / (B[2] - A[2]);
+      for (int i = 0; i < 3; i++) poly[m][i] = A[i] + (B[i] - A[i]) * s;
+      m++;
+    }
+  }
+  if (m == 0) return 0;
+  double x0 = 1e30, x1 = -1e30, y0 = 1e30, y1 = -1e30;
+  for (int k = 0; k < m; k++) {
+    const double ax = poly[k][0] / poly[k][2], by = poly[k][1] / poly[k][2];
+    const double px = (ax / c->sx + 1.0) * 0.5 * c->width - 0.5;
+    const double py = (1.0 - by / c->sy) * 0.5 * c->height - 0.5;
+    x0 = px < x0 ? px : x0; x1 = px > x1 ? px : x1;
+    y0 = py < y0 ? py : y0; y1 = py > y1 ? py : y1;
+  }
+  double fx0 = floor(x0) - 2.0, fx1 = ceil(x1) + 3.0, fy0 = floor(y0) - 2.0, fy1 = ceil(y1) + 3.0;
+  if (fx0 < 0) fx0 = 0;
+  if (fy0 < 0) fy0 = 0;
+  if (fx1 > c->width) fx1 = c->width;
+  if (fy1 > c->height) fy1 = c->height;
+  if (fx0 >= fx1 || fy0 >= fy1) return 0;
+  r[0] = (int32_t)fx0; r[1] = (int32_t)fx1; r[2] = (int32_t)fy0; r[3] = (int32_t)fy1;
+  return 1;
+}
+
+void mo_raster_pool(const mo_camera *cam, int32_t n, int32_t nc, const uint32_t *slot_chunk,
+                    const uint32_t *slot_count, const uint32_t *quads, int32_t capacity,
+                    int32_t per_slot, const float light[3], uint64_t *keys, int64_t *out_index,
+                    uint8_t *out_face, float *out_t, float *out_shade) {
+  const int64_t px_n = (int64_t)cam->width * cam->height;
+  for (int64_t i = 0; i < px_n; i++) keys[i] = ~0ull;
+  for (int32_t sl = 0; sl < capacity; sl++) {
+    const uint32_t ch = slot_chunk[sl];
+    if (ch == MO_NO_SLOT) continue;
+    const int32_t ox = (int32_t)(ch % (uint32_t)nc) * 32, oy = (int32_t)((ch / (uint32_t)nc) % (uint32_t)nc) * 32,
+                  oz = (int32_t)(ch / ((uint32_t)nc * (uint32_t)nc)) * 32;
+    const uint32_t cnt = slot_count[sl] < (uint32_t)per_slot ? slot_count[sl] : (uint32_t)per_slot;
+    for (uint32_t k = 0; k < cnt; k++) {
+      quad_geom g;
+      quad_decode(quads[(uint64_t)sl * (uint32_t)per_slot + k], ox, oy, oz, &g);
+      if (g.pos ? !(cam->eye[g.a] > g.P) : !(cam->eye[g.a] < g.P)) continue; /* back-facing */
+      int32_t r[4];
+      if (!quad_rect(cam, &g, r)) continue;
+      const uint64_t id = (uint64_t)sl * (uint32_t)per_slot + k;
+      for (int32_t py = r[2]; py < r[3]; py++)
+        for (int32_t px = r[0]; px < r[1]; px++) {
+          float o[3], d[3];
+          mo_primary_ray(cam, px, py, o, d);
+          const float t = quad_hit(&g, o, d);
+          if (t < 0.0f) continue;
+          const uint64_t key = ((uint64_t)float_bits(t) << 32) | id;
+          uint64_t *kp = &keys[(int64_t)py * cam->width + px];
+          if (key < *kp) *kp = key;
+        }
+    }
+  }
+  for (int64_t i = 0; i < px_n; i++) {
+    const uint64_t key = keys[i];
+    if (key == ~0ull) {
+      out_index[i] = -1;
+      out_face[i] = MO_FACE_MISS;
+      out_t[i] = INFINITY;
+      out_shade[i] = 0.0f;
+      continue;
+    }
+    const uint32_t id = (uint32_t)key, sl = id / (uint32_t)per_slot;
+    const uint32_t ch = slot_chunk[sl], q = quads[id];
+    quad_geom g;
+    quad_decode(q, (int32_t)(ch % (uint32_t)nc) * 32, (int32_t)((ch / (uint32_t)nc) % (uint32_t)nc) * 32,
+                (int32_t)(ch / ((uint32_t)nc * (uint32_t)nc)) * 32, &g);
+    const int32_t px = (int32_t)(i % cam->width), py = (int32_t)(i / cam->width);
+    float o[3], d[3];
+    mo_primary_ray(cam, px, py, o, d);
+    const uint32_t tb = (uint32_t)(key >> 32);
+    float t;
+    memcpy(&t, &tb, 4);
+    const float du = d[g.u] * t, dv = d[g.v] * t;
+    const float hu = o[g.u] + du, hv = o[g.v] + dv;
+    int32_t cu = (int32_t)floorf(hu), cv = (int32_t)floorf(hv);
+    cu = cu < g.U0 ? g.U0 : (cu > g.U0 + g.W - 1 ? g.U0 + g.W - 1 : cu);
+    cv = cv < g.V0 ? g.V0 : (cv > g.V0 + g.H - 1 ? g.V0 + g.H - 1 : cv);
+    int32_t cell[3];
+    cell[g.a] = g.slice;
+    cell[g.u] = cu;
+    cell[g.v] = cv;
+    const int f = (int)((q >> 25) & 7u);
+    out_index[i] = lin3(cell[0], cell[1], cell[2], n);
+    out_face[i] = (uint8_t)f;
+    out_t[i] = t;
+    const int sg = (f & 1) ? -1 : 1;
+    float lambert = sg > 0 ? light[g.a] : -light[g.a];
+    if (lambert < 0.0f) lambert = 0.0f;
+    float l = lambert * 1.0f;
+    l = 0.75f * l;
+    float shade = 0.25f + l;
+    shade = shade * 1.0f;
+    out_shade[i] = shade;
+  }
+}

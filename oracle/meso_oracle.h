@@ -34,6 +34,8 @@
  *   Shading: FP64 recomputation of every hit pixel's shade (lambert, shadow
  *     and AO occlusion by brute-force slab tests against every solid voxel).
  *   Edits / scene chunks: numpy restatement over the dense grid.
+ *   Quad visibility: float64 ray-quad brute force on single and stacked
+ *     quads; agreement with the DDA on whole frames of fully resident scenes.
  *   Chunk pool: closed-form importance values; the desired set against
  *     integer lattice counts (equal radii) and cone cases; update invariants
  *     (lookup and slots inverse maps, top-ranked loads, least important
@@ -302,6 +304,25 @@ void mo_pool_update(const mo_scene *s, const mo_pool_params *p, uint32_t stamp,
                     uint32_t *slot_chunk, uint32_t *slot_stamp, uint32_t *slot_count,
                     uint32_t *quads, int32_t *load_xyz, uint32_t *load_slot,
                     mo_pool_result *res);
+
+/* ---------------- quad visibility (the raster consumer of meshes) ---------------- */
+/* Every pixel of cam gets the nearest front-facing quad of the pool's
+ * resident slots along its primary ray (mo_primary_ray).  A quad of face f
+ * lies on the plane P = chunk origin + slice (+1 for the positive faces
+ * 0/2/4) of its axis; the ray hits it when it travels against the face
+ * normal, t = (P - o[a]) * (1 / d[a]) > 0, and the hit point
+ * h = o + d * t (per axis: d * t, then o + that) lies in [u0, u0 + w) x
+ * [v0, v0 + h).  Per pixel the smallest key (float bits of t << 32) |
+ * (slot * per_slot + k) wins.  Outputs are the raymarcher's: the voxel
+ * behind the face (index x + n(y + n z); -1 on a miss), the face, t (+inf on
+ * a miss) and the primary-only shade 0.25 + 0.75 * lambert (0 on a miss).
+ * Which pixels a quad is tested on is a conservative screen rectangle (the
+ * projected corners after clipping at 1e-3 in front of the eye, plus 2
+ * pixels); the result depends only on the per-pixel test. */
+void mo_raster_pool(const mo_camera *cam, int32_t n, int32_t nc, const uint32_t *slot_chunk,
+                    const uint32_t *slot_count, const uint32_t *quads, int32_t capacity,
+                    int32_t per_slot, const float light[3], uint64_t *keys, int64_t *out_index,
+                    uint8_t *out_face, float *out_t, float *out_shade);
 
 #ifdef __cplusplus
 }
