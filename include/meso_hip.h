@@ -473,12 +473,13 @@ typedef struct meso_raster meso_raster;
 typedef struct meso_raster_stats {
   uint32_t size;           /* sizeof(meso_raster_stats) */
   uint32_t small_quads;    /* last frame: quads of at most 64 pixels, tested by one lane */
-  uint32_t large_quads;    /* last frame: larger quads, cut into segments of 256 8x8 tiles */
-  uint32_t segments;       /* last frame: their segments */
-  uint32_t overflow_quads; /* last frame: large quads tested by one lane (segment list full) */
+  uint32_t large_quads;    /* last frame: larger quads, tested by a wave (up to 4096 pixels)
+                              or cut into segments of 16384 pixels */
+  uint32_t segments;       /* last frame: the segments */
+  uint32_t overflow_quads; /* last frame: quads tested by one lane (segment list full) */
 } meso_raster_stats;
 /* Scratch for frames up to max_width x max_height (8 B per pixel plus a
- * 32 MiB segment list). */
+ * 64 MiB segment list). */
 meso_status meso_raster_create(meso_context *ctx, int32_t max_width, int32_t max_height,
                                meso_raster **out);
 meso_status meso_raster_destroy(meso_raster *raster);

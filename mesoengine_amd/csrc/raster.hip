@@ -6,13 +6,15 @@
 // packed quads of the mesher instead.
 //
 // Work is split by quad size, on the device and without a host round trip:
-//   raster_quads_kernel: one workgroup per slot; a lane per quad culls back
-//     faces, clips the quad at 1e-3 in front of the eye and projects it to a
-//     conservative pixel rectangle (2 pixels of margin).  Quads of at most
-//     kSmall pixels are tested in place; larger ones are cut into segments of
-//     up to kSegTiles 8x8 tiles appended to a list.
+//   raster_quads_kernel: one workgroup per slot, skipped when the chunk's
+//     box projects outside the frame; a lane per quad culls back faces, clips the quad at 1e-3 in front of the eye and projects it to a
+//     conservative pixel rectangle (1 pixel of margin).  A quad of at most
+//     kSmall pixels is tested by its own lane.  A quad of at most kMedium
+//     pixels is tested by its whole wave, 64 pixels a step, the wave taking
+//     its lanes' medium quads in turn.  Larger quads are cut into segments
+//     of kSegPixels pixels appended to a list, with their rectangle.
 //   raster_segments_kernel: persistent waves, one segment at a time, a lane
-//     per pixel of a tile.
+//     per pixel.
 //   raster_resolve_kernel: a lane per pixel turns the winning key into the
 //     voxel index, face, t and shade.
 // A hit is a 64-bit key (float bits of t << 32) | (slot * per_slot + k),
@@ -29,8 +31,9 @@
 namespace meso {
 
 constexpr uint32_t kPoolNoSlot = 0xFFFFFFFFu;
-constexpr int kSmall = 64;       // pixels tested in place by the quad's lane
-constexpr int kSegTiles = 256;   // 8x8 tiles per segment
+constexpr int kSmall = 64;        // pixels tested by the quad's own lane
+constexpr int kMedium = 4096;     // pixels tested by the quad's wave
+constexpr int kSegPixels = 16384; // pixels per segment of a larger quad
 constexpr uint32_t kMaxSegments = 1u << 22;
 
 struct QuadGeom {
@@ -128,8 +131,8 @@ __device__ bool quad_rect(const DevCamera &c, const QuadGeom &g, int32_t r[4]) {
     }
   }
   if (!any) return false;
-  double fx0 = floor(b[0]) - 2.0, fx1 = ceil(b[1]) + 3.0, fy0 = floor(b[2]) - 2.0,
-         fy1 = ceil(b[3]) + 3.0;
+  double fx0 = floor(b[0]) - 1.0, fx1 = ceil(b[1]) + 2.0, fy0 = floor(b[2]) - 1.0,
+         fy1 = ceil(b[3]) + 2.0;
   fx0 = fx0 < 0.0 ? 0.0 : fx0;
   fy0 = fy0 < 0.0 ? 0.0 : fy0;
   fx1 = fx1 > c.width ? (double)c.width : fx1;
@@ -140,6 +143,44 @@ __device__ bool quad_rect(const DevCamera &c, const QuadGeom &g, int32_t r[4]) {
   r[2] = (int32_t)fy0;
   r[3] = (int32_t)fy1;
   return true;
+}
+
+// false when no part of the 32^3 chunk at (ox, oy, oz) can reach a pixel:
+// its 12 edges, clipped at z >= 1e-3, project outside the frame (or nothing
+// is in front of the eye)
+__device__ bool chunk_visible(const DevCamera &c, int32_t ox, int32_t oy, int32_t oz) {
+  double cx[8], cy[8], cz[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const double e0 = ox + ((k & 1) ? 32 : 0) - (double)c.eye[0];
+    const double e1 = oy + ((k & 2) ? 32 : 0) - (double)c.eye[1];
+    const double e2 = oz + ((k & 4) ? 32 : 0) - (double)c.eye[2];
+    cx[k] = e0 * c.right[0] + e1 * c.right[1] + e2 * c.right[2];
+    cy[k] = e0 * c.up[0] + e1 * c.up[1] + e2 * c.up[2];
+    cz[k] = e0 * c.fwd[0] + e1 * c.fwd[1] + e2 * c.fwd[2];
+  }
+  const double zn = 1e-3;
+  double b[4] = {1e30, -1e30, 1e30, -1e30};
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    if (cz[k] >= zn) {
+      bound_point(c, cx[k], cy[k], cz[k], b);
+      any = true;
+    }
+#pragma unroll
+    for (int bit = 1; bit < 8; bit <<= 1) {  // the edges k -> k | bit
+      if (k & bit) continue;
+      const int l = k | bit;
+      if ((cz[k] >= zn) != (cz[l] >= zn)) {
+        const double t = (zn - cz[k]) / (cz[l] - cz[k]);
+        bound_point(c, cx[k] + (cx[l] - cx[k]) * t, cy[k] + (cy[l] - cy[k]) * t, zn, b);
+        any = true;
+      }
+    }
+  }
+  if (!any) return false;
+  return b[1] >= -2.0 && b[0] <= c.width + 1.0 && b[3] >= -2.0 && b[2] <= c.height + 1.0;
 }
 
 __device__ __forceinline__ void test_pixel(const DevCamera &cam, const QuadGeom &g, int32_t px,
@@ -154,9 +195,18 @@ __device__ __forceinline__ void test_pixel(const DevCamera &cam, const QuadGeom 
 
 struct RasterScratch {
   unsigned long long *keys;  // width * height
-  uint2 *segs;               // (quad id, first tile)
-  unsigned int *counters;    // [0] segments, [1] overflowed quads, [2] small quads, [3] large quads
+  uint4 *segs;               // quad id, first pixel, x0 | y0 << 16, width | height << 16
+  unsigned int *counters;    // [0] segments, [1] overflowed quads, [2] small, [3] medium + large
 };
+
+// pixels [j0, j1) of a quad's rectangle, row-major, a lane per pixel
+__device__ __forceinline__ void wave_pixels(const DevCamera &cam, const QuadGeom &g, int32_t x0,
+                                            int32_t y0, int32_t rw, uint32_t j0, uint32_t j1,
+                                            uint64_t id, unsigned long long *keys) {
+  for (uint32_t j = j0 + (threadIdx.x & 63u); j < j1; j += 64u)
+    test_pixel(cam, g, x0 + (int32_t)(j % (uint32_t)rw), y0 + (int32_t)(j / (uint32_t)rw), id,
+               keys);
+}
 
 __global__ void __launch_bounds__(256) raster_clear_kernel(RasterScratch s, uint64_t npx) {
   const uint64_t i = block_id() * 256u + threadIdx.x;
@@ -171,56 +221,80 @@ __global__ void __launch_bounds__(256) raster_quads_kernel(DevCamera cam, const 
   const uint32_t sl = blockIdx.x;
   const uint32_t chunk = slot_chunk[sl];
   if (chunk == kPoolNoSlot) return;
+  // the whole chunk outside the frame: none of its quads can be hit
+  if (!chunk_visible(cam, (int32_t)(chunk % nc) * 32, (int32_t)((chunk / nc) % nc) * 32,
+                     (int32_t)(chunk / (nc * nc)) * 32))
+    return;
   const uint32_t cnt = slot_count[sl] < per_slot ? slot_count[sl] : per_slot;
   const float eye[3] = {cam.eye[0], cam.eye[1], cam.eye[2]};
-  for (uint32_t k = threadIdx.x; k < cnt; k += 256u) {
+  // the trip count is the same for every lane, so the waves stay whole for
+  // the wave-wide steps below
+  for (uint32_t base = 0; base < cnt; base += 256u) {
+    const uint32_t k = base + threadIdx.x;
     const uint64_t id = (uint64_t)sl * per_slot + k;
-    const QuadGeom g = quad_decode(quads[id], chunk, nc);
-    const float e = pick3(eye, g.a);
-    if (g.pos ? !(e > g.P) : !(e < g.P)) continue;  // back-facing for every pixel
-    int32_t r[4];
-    if (!quad_rect(cam, g, r)) continue;
+    int32_t r[4] = {0, 0, 0, 0};
+    bool live = false;
+    QuadGeom g;
+    if (k < cnt) {
+      g = quad_decode(quads[id], chunk, nc);
+      const float e = pick3(eye, g.a);
+      // back-facing quads cannot be hit from this eye
+      live = (g.pos ? e > g.P : e < g.P) && quad_rect(cam, g, r);
+    }
     const int32_t rw = r[1] - r[0], rh = r[3] - r[2];
-    if (rw * rh <= kSmall) {
+    const int32_t area = live ? rw * rh : 0;
+    if (live && area <= kSmall) {
       atomicAdd(&s.counters[2], 1u);
       for (int32_t py = r[2]; py < r[3]; py++)
         for (int32_t px = r[0]; px < r[1]; px++) test_pixel(cam, g, px, py, id, s.keys);
-      continue;
     }
-    const uint32_t tiles = (uint32_t)(((rw + 7) >> 3) * ((rh + 7) >> 3));
-    const uint32_t nseg = (tiles + kSegTiles - 1) / kSegTiles;
-    atomicAdd(&s.counters[3], 1u);
-    const uint32_t j = atomicAdd(&s.counters[0], nseg);
-    if (j + nseg <= kMaxSegments) {
-      for (uint32_t q = 0; q < nseg; q++) s.segs[j + q] = make_uint2((uint32_t)id, q * kSegTiles);
-    } else {
-      // the segment list is full: test the quad's pixels here (slow, never wrong)
-      atomicAdd(&s.counters[1], 1u);
-      for (int32_t py = r[2]; py < r[3]; py++)
-        for (int32_t px = r[0]; px < r[1]; px++) test_pixel(cam, g, px, py, id, s.keys);
+    const bool medium = live && area > kSmall && area <= kMedium;
+    if (live && area > kMedium) {
+      const uint32_t nseg = ((uint32_t)area + kSegPixels - 1) / kSegPixels;
+      atomicAdd(&s.counters[3], 1u);
+      const uint32_t j = atomicAdd(&s.counters[0], nseg);
+      if (j + nseg <= kMaxSegments) {
+        for (uint32_t q = 0; q < nseg; q++)
+          s.segs[j + q] = make_uint4((uint32_t)id, q * kSegPixels,
+                                     (uint32_t)r[0] | ((uint32_t)r[2] << 16),
+                                     (uint32_t)rw | ((uint32_t)rh << 16));
+      } else {
+        // the segment list is full: test the quad's pixels here (slow, never wrong)
+        atomicAdd(&s.counters[1], 1u);
+        for (int32_t py = r[2]; py < r[3]; py++)
+          for (int32_t px = r[0]; px < r[1]; px++) test_pixel(cam, g, px, py, id, s.keys);
+      }
+    }
+    // the wave takes its medium quads one at a time, 64 pixels a step
+    uint64_t todo = __ballot(medium);
+    if (medium) atomicAdd(&s.counters[3], 1u);
+    while (todo) {
+      const int L = __ffsll((unsigned long long)todo) - 1;
+      todo &= todo - 1ull;
+      const uint32_t mk = (uint32_t)__shfl((int)k, L, 64);
+      const int32_t mx0 = __shfl(r[0], L, 64), my0 = __shfl(r[2], L, 64);
+      const int32_t mrw = __shfl(rw, L, 64), marea = __shfl(area, L, 64);
+      const uint64_t mid = (uint64_t)sl * per_slot + mk;
+      const QuadGeom mg = quad_decode(quads[mid], chunk, nc);
+      wave_pixels(cam, mg, mx0, my0, mrw, 0u, (uint32_t)marea, mid, s.keys);
     }
   }
 }
 
-// persistent waves over the segments: lane = pixel of an 8x8 tile
+// persistent waves over the segments of the large quads
 __global__ void __launch_bounds__(64) raster_segments_kernel(DevCamera cam, const uint32_t *slot_chunk,
                                                              const uint32_t *quads, uint32_t per_slot,
                                                              uint32_t nc, RasterScratch s) {
   const uint32_t n = s.counters[0] < kMaxSegments ? s.counters[0] : kMaxSegments;
-  const uint32_t lane = threadIdx.x;
   for (uint32_t j = blockIdx.x; j < n; j += gridDim.x) {
-    const uint2 seg = s.segs[j];
+    const uint4 seg = s.segs[j];
     const uint32_t id = seg.x;
     const QuadGeom g = quad_decode(quads[id], slot_chunk[id / per_slot], nc);
-    int32_t r[4];
-    if (!quad_rect(cam, g, r)) continue;  // the same rectangle as when the segment was made
-    const uint32_t tw = (uint32_t)((r[1] - r[0] + 7) >> 3), th = (uint32_t)((r[3] - r[2] + 7) >> 3);
-    const uint32_t end = seg.y + kSegTiles < tw * th ? seg.y + kSegTiles : tw * th;
-    for (uint32_t tile = seg.y; tile < end; tile++) {
-      const int32_t px = r[0] + (int32_t)((tile % tw) * 8u + (lane & 7u));
-      const int32_t py = r[2] + (int32_t)((tile / tw) * 8u + (lane >> 3));
-      if (px < r[1] && py < r[3]) test_pixel(cam, g, px, py, id, s.keys);
-    }
+    const int32_t x0 = (int32_t)(seg.z & 0xFFFFu), y0 = (int32_t)(seg.z >> 16);
+    const int32_t rw = (int32_t)(seg.w & 0xFFFFu), rh = (int32_t)(seg.w >> 16);
+    const uint32_t area = (uint32_t)rw * (uint32_t)rh;
+    const uint32_t end = seg.y + kSegPixels < area ? seg.y + kSegPixels : area;
+    wave_pixels(cam, g, x0, y0, rw, seg.y, end, id, s.keys);
   }
 }
 
@@ -295,7 +369,7 @@ meso_status meso_raster_create(meso_context *ctx, int32_t max_width, int32_t max
   r->max_height = max_height;
   const uint64_t npx = (uint64_t)max_width * (uint64_t)max_height;
   if (hipMalloc((void **)&r->s.keys, npx * 8) != hipSuccess ||
-      hipMalloc((void **)&r->s.segs, (uint64_t)kMaxSegments * 8) != hipSuccess ||
+      hipMalloc((void **)&r->s.segs, (uint64_t)kMaxSegments * 16) != hipSuccess ||
       hipMalloc((void **)&r->s.counters, 16) != hipSuccess) {
     (void)hipGetLastError();
     if (r->s.keys) (void)hipFree(r->s.keys);

@@ -14,6 +14,10 @@ Two measurements:
     start).
   sample: the 512^3 heightfield with a 4,096-slot pool and radius 16, two
     updates; the same updates run on the CPU oracle in bench.py's CPU leg.
+  raster: the fly-through pool's resident quads drawn at 1920x1080
+    (meso_raster_pool), timed beside the raymarcher's primary rays on the
+    same camera; outside the resident radius the raymarcher sees geometry
+    the pool does not hold, so agreement counts every hit pixel.
 """
 import time
 
@@ -79,6 +83,7 @@ def run(ctx, updates=24, warmup=72, log=print):
     log("streaming: fly-through %.3f ms/update (median), %.0f loads, %.0f candidates, "
         "%.0f Mvox/s" % (med, fly["loads_per_update"], fly["candidates_per_update"],
                          fly["mvox_per_s"]))
+    raster = draw_leg(ctx, sc, pool, *fly_path(warmup + updates - 1), log=log)
     pool.destroy()
     # cold start: every slot in one update
     cold = dict(FLY, max_loads=FLY["capacity"])
@@ -111,4 +116,47 @@ def run(ctx, updates=24, warmup=72, log=print):
                   mvox_per_s=round(sample_loads * 32768 / (sample_ms * 1e-3) / 1e6, 1),
                   workload="512^3 heightfield brick map, %d slots, radius %d, %d updates" % (
                       SAMPLE["capacity"], SAMPLE["forward_radius"], len(SAMPLE_PATH)))
-    return dict(fly_through=fly, cold_start=fill, sample=sample)
+    return dict(fly_through=fly, cold_start=fill, sample=sample, raster=raster)
+
+
+def draw_leg(ctx, sc, pool, cam_chunk, fwd, width=1920, height=1080, frames=10, log=print):
+    """The pool's resident quads drawn at 1920x1080 from the camera of the
+    last fly-through update (meso_raster_pool), against the raymarcher's
+    primary rays (flags 0) on the same camera and scene."""
+    eye = [c * 32 + 16.0 for c in cam_chunk]
+    target = [eye[i] + fwd[i] for i in range(3)]
+    cam = R.camera(eye, target, [0.0, 1.0, 0.0], 70.0, width, height)
+    sh = R.shading(W.LIGHT)
+    rast = R.Raster(ctx, width, height)
+    tg = R.RayTargets(ctx, width, height, index_bytes=8)
+    rast.draw(pool, cam, sh, tg)  # untimed first use
+    ctx.sync()
+
+    def timed(fn):
+        e0, e1 = R.Event(ctx), R.Event(ctx)
+        e0.record()
+        for _ in range(frames):
+            fn()
+        e1.record()
+        e1.sync()
+        return e0.elapsed_ms(e1) / frames
+
+    ms = timed(lambda: rast.draw(pool, cam, sh, tg))
+    st = rast.stats()
+    gi, gf, _, _ = tg.results()
+    tr = R.RayTargets(ctx, width, height, index_bytes=8)
+    R.raymarch(ctx, sc, cam, 0, sh, tr)
+    ctx.sync()
+    ms_rm = timed(lambda: R.raymarch(ctx, sc, cam, 0, sh, tr))
+    di, df, _, _ = tr.results()
+    hit = (gi >= 0) | (di >= 0)
+    agree = float(((gi == di) & (gf == df))[hit].mean()) if hit.any() else 1.0
+    rast.destroy()
+    out = dict(ms=round(ms, 4), raymarch_primary_ms=round(ms_rm, 4),
+               resident_quads=pool.stats()["n_quads"], hit_px=int((gi >= 0).sum()),
+               agree_with_raymarch=round(agree, 5), **st)
+    log("streaming: raster %.3f ms at %dx%d (%d small, %d large quads, %d segments); "
+        "raymarch primary %.3f ms; agree %.4f" % (ms, width, height, st["small_quads"],
+                                                   st["large_quads"], st["segments"], ms_rm,
+                                                   agree))
+    return out

@@ -1457,7 +1457,7 @@ static int quad_rect(const mo_camera *c, const quad_geom *g, int32_t r[4]) {
     x0 = px < x0 ? px : x0; x1 = px > x1 ? px : x1;
     y0 = py < y0 ? py : y0; y1 = py > y1 ? py : y1;
   }
-  double fx0 = floor(x0) - 2.0, fx1 = ceil(x1) + 3.0, fy0 = floor(y0) - 2.0, fy1 = ceil(y1) + 3.0;
+  double fx0 = floor(x0) - 1.0, fx1 = ceil(x1) + 2.0, fy0 = floor(y0) - 1.0, fy1 = ceil(y1) + 2.0;
   if (fx0 < 0) fx0 = 0;
   if (fy0 < 0) fy0 = 0;
   if (fx1 > c->width) fx1 = c->width;

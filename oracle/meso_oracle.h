@@ -317,8 +317,8 @@ void mo_pool_update(const mo_scene *s, const mo_pool_params *p, uint32_t stamp,
  * behind the face (index x + n(y + n z); -1 on a miss), the face, t (+inf on
  * a miss) and the primary-only shade 0.25 + 0.75 * lambert (0 on a miss).
  * Which pixels a quad is tested on is a conservative screen rectangle (the
- * projected corners after clipping at 1e-3 in front of the eye, plus 2
- * pixels); the result depends only on the per-pixel test. */
+ * projected corners after clipping at 1e-3 in front of the eye, plus 1
+ * pixel); the result depends only on the per-pixel test. */
 void mo_raster_pool(const mo_camera *cam, int32_t n, int32_t nc, const uint32_t *slot_chunk,
                     const uint32_t *slot_count, const uint32_t *quads, int32_t capacity,
                     int32_t per_slot, const float light[3], uint64_t *keys, int64_t *out_index,

@@ -1,7 +1,8 @@
 """Quad visibility on the GPU against the oracle's: the chunk pool's quads
 drawn by meso_raster_pool give the oracle's index, face, t and shade bit for
-bit (small quads tested by one lane, large ones cut into segments, quads
-clipped at the eye), and agree with the GPU raymarcher's primary hits."""
+bit (small quads tested by one lane, medium ones by a wave, large ones cut
+into segments, quads clipped at the eye), and agree with the GPU
+raymarcher's primary hits.  A close-up frame makes quads of every size."""
 import ctypes as C
 
 import numpy as np
@@ -19,12 +20,23 @@ CASES = [(L.SCENE_BRICKMAP_U16, L.GEN_HEIGHTFIELD, 128, 2, (-30.0, 90.0, -20.0),
          (L.SCENE_TWO_LEVEL_U16, L.GEN_CAVES, 256, 5, (128.0, 140.0, 120.0), (200.0, 100.0, 180.0),
           320, 200),
          (L.SCENE_TWO_LEVEL_U16, L.GEN_CAVES, 256, 5, (-40.0, 300.0, -40.0), (128.0, 100.0, 128.0),
-          203, 117)]
+          203, 117),
+         # a floor slab (y < 8) with a block on it, seen from just above: 32x32
+         # greedy quads that cover thousands of pixels
+         (L.SCENE_BRICKMAP_U16, "slab", 64, 0, (20.0, 9.5, 6.0), (40.0, 4.0, 50.0), 400, 300)]
+
+
+def _slab(ctx, n):
+    g = np.zeros((n, n, n), np.uint16)  # (z, y, x)
+    g[:, :8, :] = 1
+    g[20:28, 8:14, 30:40] = 2
+    slots, masks, vox = O.brickmap_from_dense(g)
+    return ctx.scene_brickmap(n, slots, masks, vox)
 
 
 @pytest.mark.parametrize("kind,gen,n,seed,eye,target,w,h", CASES)
 def test_raster_matches_oracle_and_raymarch(ctx, kind, gen, n, seed, eye, target, w, h):
-    sc = ctx.scene_generate(gen, kind, n, seed)
+    sc = _slab(ctx, n) if gen == "slab" else ctx.scene_generate(gen, kind, n, seed)
     nc = -(-n // 32)
     R_ = int(np.ceil(np.sqrt(3) * nc)) + 1
     per = 65536
@@ -41,8 +53,11 @@ def test_raster_matches_oracle_and_raymarch(ctx, kind, gen, n, seed, eye, target
     ctx.sync()
     gi, gf, gt, gs = tg.results()
     rs = rast.stats()
-    assert rs["small_quads"] > 0 and rs["large_quads"] > 0 and rs["segments"] >= rs["large_quads"]
-    assert rs["overflow_quads"] == 0
+    assert rs["small_quads"] + rs["large_quads"] > 0 and rs["overflow_quads"] == 0
+    if gen == "slab":
+        assert rs["segments"] > 0, rs  # quads of more than 4096 pixels
+    else:
+        assert rs["small_quads"] > 0 and rs["large_quads"] > 0, rs
     ocam = O.camera(list(eye), list(target), [0.0, 1.0, 0.0], 70.0, w, h)
     oi, of, ot, os_, _ = O.raster_pool(ocam, n, nc, st["slot_chunk"], st["slot_count"],
                                        st["quads"], per, O.shading(W.LIGHT).light)
