@@ -402,6 +402,10 @@ typedef struct meso_pool_desc {
   int32_t max_loads;       /* chunks loaded per update, 1..capacity
                               (MaxSyncedLoadChunkCount + MaxUnsyncedLoadChunkCount: 256) */
   uint32_t mesh_mode;      /* MESO_MESH_CULLED or MESO_MESH_GREEDY */
+  int32_t bake_directions; /* BakeVisibilityViewNum: 0 = the exact forward vector; 2..65536 =
+                              the nearest of that many Fibonacci-sphere directions
+                              (GetFibonacciSphere, VoxelMathHelper.h:48-71; the reference's
+                              default is 256), used for the whole update */
 } meso_pool_desc;
 typedef struct meso_pool_stats {
   uint32_t size;          /* sizeof(meso_pool_stats) */

@@ -93,7 +93,8 @@ class HostOutputs(C.Structure):
 class PoolDesc(C.Structure):
     _fields_ = [("size", C.c_uint32), ("capacity", C.c_int32), ("quads_per_slot", C.c_int32),
                 ("forward_radius", C.c_int32), ("backward_radius", C.c_int32),
-                ("view_angle", C.c_float), ("max_loads", C.c_int32), ("mesh_mode", C.c_uint32)]
+                ("view_angle", C.c_float), ("max_loads", C.c_int32), ("mesh_mode", C.c_uint32),
+                ("bake_directions", C.c_int32)]
 
 
 class PoolStats(C.Structure):

@@ -14,6 +14,7 @@
 // oracle (-ffp-contract=off), so the ranking is bit-identical to it.
 #include <cmath>
 #include <cstring>  // before rocprim, whose texture iterator calls memset
+#include <vector>
 
 #include <rocprim/rocprim.hpp>
 
@@ -285,7 +286,30 @@ struct meso_chunk_pool {
   void *sort_tmp = nullptr;
   size_t sort_bytes = 0;
   uint64_t bytes = 0;
+  std::vector<float> dirs;  // baked view directions, 3 per direction (empty: exact)
 };
+
+// unit vector v / length, length = sqrt((x*x + y*y) + z*z), in float
+static void normalize_f(const float v[3], float out[3]) {
+  float ss = v[0] * v[0] + v[1] * v[1];
+  ss = ss + v[2] * v[2];
+  const float len = sqrtf(ss);
+  for (int k = 0; k < 3; k++) out[k] = v[k] / len;
+}
+
+// GetFibonacciSphere (Runtimes/Helper/VoxelMathHelper.h:48-71): y = 1 - 2i/(n-1),
+// radius sqrt(1 - y^2), angle i * pi * (sqrt(5) - 1), in double, then float
+static void bake_directions(int32_t n, std::vector<float> &dirs) {
+  dirs.resize((size_t)n * 3);
+  const double phi = 3.14159265358979323846 * (std::sqrt(5.0) - 1.0);
+  for (int32_t i = 0; i < n; i++) {
+    const double y = 1.0 - ((double)i / (double)(n - 1)) * 2.0;
+    const double r = std::sqrt(1.0 - y * y);
+    const double th = phi * (double)i;
+    const float v[3] = {(float)(std::cos(th) * r), (float)y, (float)(std::sin(th) * r)};
+    normalize_f(v, &dirs[(size_t)i * 3]);
+  }
+}
 
 static void pool_free(meso_chunk_pool *p) {
   void *ptrs[] = {p->lookup, p->slot_chunk, p->slot_stamp, p->slot_count, p->quads, p->counters,
@@ -340,7 +364,11 @@ meso_status meso_chunk_pool_create(meso_context *ctx, meso_scene *scene, const m
              "max_loads %d not in 1..capacity", d.max_loads);
   MESO_CHECK(d.mesh_mode == MESO_MESH_CULLED || d.mesh_mode == MESO_MESH_GREEDY,
              MESO_ERR_ARGUMENT_OUT_OF_RANGE, "unknown mesh mode %u", d.mesh_mode);
+  MESO_CHECK(d.bake_directions == 0 || (d.bake_directions >= 2 && d.bake_directions <= 65536),
+             MESO_ERR_ARGUMENT_OUT_OF_RANGE, "bake_directions %d not 0 or 2..65536",
+             d.bake_directions);
   meso_chunk_pool *p = new meso_chunk_pool;
+  if (d.bake_directions) bake_directions(d.bake_directions, p->dirs);
   p->ctx = ctx;
   p->scene = scene;
   p->desc = d;
@@ -432,7 +460,27 @@ meso_status meso_chunk_pool_update(meso_chunk_pool *pool, const int32_t camera_c
   MESO_CHECK(ss > 0.0f && std::isfinite(ss), MESO_ERR_ARGUMENT_OUT_OF_RANGE,
              "forward vector has no direction");
   const float len = sqrtf(ss);
-  const float f0 = forward[0] / len, f1 = forward[1] / len, f2 = forward[2] / len;
+  float f0 = forward[0] / len, f1 = forward[1] / len, f2 = forward[2] / len;
+  if (!pool->dirs.empty()) {
+    // the nearest baked direction (TNearestMap::Query, NearestMap.h:36-50;
+    // the first one on ties)
+    const size_t nd = pool->dirs.size() / 3;
+    size_t best = 0;
+    float bd = INFINITY;
+    for (size_t i = 0; i < nd; i++) {
+      const float dx = pool->dirs[3 * i] - f0, dy = pool->dirs[3 * i + 1] - f1,
+                  dz = pool->dirs[3 * i + 2] - f2;
+      float dd = dx * dx + dy * dy;
+      dd = dd + dz * dz;
+      if (dd < bd) {
+        bd = dd;
+        best = i;
+      }
+    }
+    f0 = pool->dirs[3 * best];
+    f1 = pool->dirs[3 * best + 1];
+    f2 = pool->dirs[3 * best + 2];
+  }
   const hipStream_t s = stream_of(pool->ctx, stream);
   const PoolGeom &g = pool->geom;
   const int32_t cx = camera_chunk[0], cy = camera_chunk[1], cz = camera_chunk[2];

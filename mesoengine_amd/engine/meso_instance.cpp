@@ -135,6 +135,7 @@ void FHipVoxelInstance::InitializeBegin() {
     pd.view_angle = c.ViewChunkAngle;
     pd.max_loads = (int32_t)(c.MaxSyncedLoadChunkCount + c.MaxUnsyncedLoadChunkCount);
     pd.mesh_mode = desc_.mesh_mode;
+    pd.bake_directions = (int32_t)c.BakeVisibilityViewNum;
     Check(meso_chunk_pool_create(ctx_, scene_, &pd, &pool_));
   }
 }

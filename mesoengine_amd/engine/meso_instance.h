@@ -46,6 +46,7 @@ struct FVoxelSceneConfig {
   uint32_t MaxSyncedLoadChunkCount = 0;
   uint32_t MaxUnsyncedLoadChunkCount = 256;
   float ViewChunkAngle = 120.0f;
+  uint32_t BakeVisibilityViewNum = 256;
   uint32_t ChunkQuadCapacity = 8192;
   float GetChunkSize() const { return (float)ChunkResolution * BlockSize; }
 };

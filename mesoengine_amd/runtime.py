@@ -356,10 +356,11 @@ class ChunkPool:
     fixed slots (meso_chunk_pool_*)."""
 
     def __init__(self, ctx, scene, capacity=16384, quads_per_slot=8192, forward_radius=24,
-                 backward_radius=6, view_angle=120.0, max_loads=256, mode=L.MESH_GREEDY):
+                 backward_radius=6, view_angle=120.0, max_loads=256, mode=L.MESH_GREEDY,
+                 bake_directions=0):
         self.ctx, self.scene = ctx, scene
         self.desc = L.PoolDesc(C.sizeof(L.PoolDesc), capacity, quads_per_slot, forward_radius,
-                               backward_radius, view_angle, max_loads, mode)
+                               backward_radius, view_angle, max_loads, mode, bake_directions)
         h = C.c_void_p()
         check(ctx.lib.meso_chunk_pool_create(ctx.h, scene.h, byref(self.desc), C.byref(h)))
         self.h = h

@@ -4,7 +4,8 @@ Two measurements:
   fly-through: the 2048^3 two-level cave volume; the camera circles the
     middle of the volume about one chunk per update, and each update ranks
     the chunks of
-    its view sphere (radius 24 chunks forward, 6 backward, 120 degree cone),
+    its view sphere (radius 24 chunks forward, 6 backward, 120 degree cone,
+    the view snapped to the nearest of 256 baked directions),
     loads at most 256 of them (the reference's default load limit) into a
     16,384-slot pool (the reference's default MaxChunkCount), evicting the
     least important, and greedy-meshes them into their slots.  The untimed
@@ -26,7 +27,7 @@ from . import runtime as R
 from . import workloads as W
 
 FLY = dict(capacity=16384, quads_per_slot=8192, forward_radius=24, backward_radius=6,
-           view_angle=120.0, max_loads=256)
+           view_angle=120.0, max_loads=256, bake_directions=256)
 SAMPLE = dict(capacity=4096, quads_per_slot=8192, forward_radius=16, backward_radius=6,
               view_angle=120.0, max_loads=256)
 SAMPLE_PATH = [((8, 4, 8), (1.0, -0.2, 0.5)), ((9, 4, 8), (1.0, -0.2, 0.6))]

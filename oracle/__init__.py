@@ -122,6 +122,8 @@ def lib():
             "mo_scene_chunk": (None, [P, C.c_int32, C.c_int32, C.c_int32, P]),
             "mo_pool_view_threshold": (C.c_float, [C.c_float]),
             "mo_pool_forward": (None, [P, P]),
+            "mo_pool_bake_directions": (None, [C.c_int32, P]),
+            "mo_pool_snap": (None, [P, C.c_int32, P, P]),
             "mo_chunk_importance": (C.c_float, [C.c_int32] * 3 + [P]),
             "mo_chunk_desired": (C.c_int, [C.c_int32] * 3 + [P, P]),
             "mo_chunk_nonempty": (C.c_int, [P] + [C.c_int32] * 3),
@@ -447,6 +449,20 @@ def pool_forward(raw):
     return f
 
 
+def pool_bake_directions(n):
+    d = np.zeros(3 * n, np.float32)
+    lib().mo_pool_bake_directions(n, _p(d))
+    return d.reshape(n, 3)
+
+
+def pool_snap(dirs, f):
+    d = np.ascontiguousarray(dirs, np.float32)
+    fv = np.ascontiguousarray(f, np.float32)
+    out = np.zeros(3, np.float32)
+    lib().mo_pool_snap(_p(d), len(d), _p(fv), _p(out))
+    return out
+
+
 def chunk_importance(dx, dy, dz, f):
     f = np.ascontiguousarray(f, np.float32)
     return lib().mo_chunk_importance(dx, dy, dz, _p(f))
@@ -470,8 +486,11 @@ def chunk_nonempty(scene, cx, cy, cz):
 class OraclePool:
     """The chunk pool's state (lookup table, slots, quads) and its updates."""
 
-    def __init__(self, scene, params):
+    def __init__(self, scene, params, bake_directions=0):
+        """bake_directions > 0: the view is the nearest of that many baked
+        directions (the reference's BakeVisibilityViewNum)"""
         self.scene, self.p = scene, params
+        self.dirs = pool_bake_directions(bake_directions) if bake_directions else None
         self.nc = -(-scene.n // 32)
         self.lookup = np.full(self.nc ** 3, NO_SLOT, np.uint32)
         cap = params.capacity
@@ -487,6 +506,8 @@ class OraclePool:
         """one update; returns (n_candidates, n_loads, n_evicted, loads [(x, y, z, slot)])"""
         self.stamp = (self.stamp + 1) & 0xFFFFFFFF
         f = pool_forward(forward_raw)
+        if self.dirs is not None:
+            f = pool_snap(self.dirs, f)
         cam = np.ascontiguousarray(cam_chunk, np.int32)
         r = PoolResult()
         lib().mo_pool_update(self.scene.ptr, C.byref(self.p), self.stamp, _p(cam), _p(f),

@@ -1233,6 +1233,32 @@ void mo_pool_forward(const float raw[3], float f[3]) {
   for (int i = 0; i < 3; i++) f[i] = raw[i] / len;
 }
 
+void mo_pool_bake_directions(int32_t n, float *dirs) {
+  const double phi = 3.14159265358979323846 * (sqrt(5.0) - 1.0);
+  for (int32_t i = 0; i < n; i++) {
+    const double y = 1.0 - ((double)i / (double)(n - 1)) * 2.0;
+    const double r = sqrt(1.0 - y * y);
+    const double th = phi * (double)i;
+    const float v[3] = {(float)(cos(th) * r), (float)y, (float)(sin(th) * r)};
+    mo_pool_forward(v, dirs + 3 * i);
+  }
+}
+
+void mo_pool_snap(const float *dirs, int32_t n, const float f[3], float out[3]) {
+  int32_t best = 0;
+  float bd = INFINITY;
+  for (int32_t i = 0; i < n; i++) {
+    const float dx = dirs[3 * i] - f[0], dy = dirs[3 * i + 1] - f[1], dz = dirs[3 * i + 2] - f[2];
+    float d = dx * dx + dy * dy;
+    d = d + dz * dz;
+    if (d < bd) {
+      bd = d;
+      best = i;
+    }
+  }
+  for (int k = 0; k < 3; k++) out[k] = dirs[3 * best + k];
+}
+
 /* length of an integer offset and the dot of its direction with f */
 static float offset_dir_dot(int32_t dx, int32_t dy, int32_t dz, const float f[3], float *len) {
   const float x = (float)dx, y = (float)dy, z = (float)dz;

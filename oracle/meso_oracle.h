@@ -281,6 +281,16 @@ typedef struct {
 } mo_pool_result;
 float mo_pool_view_threshold(float view_angle_deg);
 void mo_pool_forward(const float raw[3], float f[3]);
+/* FVoxelMathHelper::GetFibonacciSphere (Runtimes/Helper/VoxelMathHelper.h:48-71):
+ * n >= 2 unit directions, y = 1 - 2i/(n-1), radius sqrt(1 - y^2), angle
+ * i * pi * (sqrt(5) - 1), evaluated in double, rounded to float and
+ * normalised in float (length = sqrt((x*x + y*y) + z*z), v / length). */
+void mo_pool_bake_directions(int32_t n, float *dirs);
+/* The baked direction nearest to the unit forward f (TNearestMap::Query,
+ * Runtimes/Voxel/Spatial/NearestMap.h:36-50: Euclidean nearest; the first
+ * on ties): the view the reference's BakeVisibilityByView
+ * (ChunkManagerHelper.h:186-219) answers with. */
+void mo_pool_snap(const float *dirs, int32_t n, const float f[3], float out[3]);
 /* FImportanceComputeInfo::CalculateChunkImportance (ChunkManagerHelper.h:21-43) */
 float mo_chunk_importance(int32_t dx, int32_t dy, int32_t dz, const float f[3]);
 /* GetDesiredShowChunkLocationByView's acceptance (ChunkManagerHelper.h:98-135) */

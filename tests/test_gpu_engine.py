@@ -175,7 +175,7 @@ def test_instance_chunk_streaming_matches_oracle(tmp_path):
     osc = O.two_level_scene(n, top, slots, masks, vox)
     pool = O.OraclePool(osc, O.pool_params(forward_radius=4, backward_radius=4, view_angle=120.0,
                                            capacity=64, quads_per_slot=pd["q"], max_loads=16,
-                                           greedy=True))
+                                           greedy=True), bake_directions=256)
     loads = []
     for f in range(frames):
         s, xyz = d["sets"][f]

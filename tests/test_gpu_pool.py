@@ -48,17 +48,19 @@ def _compare(pool, opool, step):
     return st, g
 
 
-@pytest.mark.parametrize("kind,gen,n,seed,mode", [
-    (L.SCENE_BRICKMAP_U16, L.GEN_HEIGHTFIELD, 256, 2, L.MESH_GREEDY),
-    (L.SCENE_BRICKMAP_U16, L.GEN_HEIGHTFIELD, 256, 2, L.MESH_CULLED),
-    (L.SCENE_TWO_LEVEL_U16, L.GEN_CAVES, 256, 5, L.MESH_GREEDY)])
-def test_pool_updates_match_oracle(ctx, kind, gen, n, seed, mode):
+@pytest.mark.parametrize("kind,gen,n,seed,mode,bake", [
+    (L.SCENE_BRICKMAP_U16, L.GEN_HEIGHTFIELD, 256, 2, L.MESH_GREEDY, 0),
+    (L.SCENE_BRICKMAP_U16, L.GEN_HEIGHTFIELD, 256, 2, L.MESH_CULLED, 0),
+    (L.SCENE_TWO_LEVEL_U16, L.GEN_CAVES, 256, 5, L.MESH_GREEDY, 0),
+    (L.SCENE_TWO_LEVEL_U16, L.GEN_CAVES, 256, 5, L.MESH_GREEDY, 256)])
+def test_pool_updates_match_oracle(ctx, kind, gen, n, seed, mode, bake):
     sc = ctx.scene_generate(gen, kind, n, seed)
     osc, _, _ = _oracle_scene(kind, gen, n, seed)
     kw = dict(capacity=48, quads_per_slot=3000, forward_radius=4, backward_radius=2,
               view_angle=100.0, max_loads=12)
-    pool = R.ChunkPool(ctx, sc, mode=mode, **kw)
-    opool = O.OraclePool(osc, O.pool_params(greedy=mode == L.MESH_GREEDY, **kw))
+    pool = R.ChunkPool(ctx, sc, mode=mode, bake_directions=bake, **kw)
+    opool = O.OraclePool(osc, O.pool_params(greedy=mode == L.MESH_GREEDY, **kw),
+                         bake_directions=bake)
     loads_seen = evicted_seen = truncated_seen = 0
     for step, (cam, fwd) in enumerate(PATH):
         pool.update(cam, fwd)
@@ -119,7 +121,8 @@ def test_pool_argument_checks(ctx):
                 view_angle=90.0, max_loads=16, mesh_mode=L.MESH_GREEDY)
     for bad in [dict(capacity=0), dict(quads_per_slot=0), dict(forward_radius=0),
                 dict(backward_radius=5), dict(view_angle=0.0), dict(view_angle=float("nan")),
-                dict(max_loads=65), dict(mesh_mode=7)]:
+                dict(max_loads=65), dict(mesh_mode=7), dict(bake_directions=1),
+                dict(bake_directions=70000)]:
         kw = dict(good, **bad)
         d = L.PoolDesc(C.sizeof(L.PoolDesc), **kw)
         assert ctx.lib.meso_chunk_pool_create(ctx.h, sc.h, L.byref(d), C.byref(h)) == L.ERR_ARG, bad

@@ -98,6 +98,50 @@ def test_desired_against_float64_rule(fwd, R, RB, angle):
     assert checked > 0.95 * (2 * R + 1) ** 3
 
 
+@pytest.mark.parametrize("n", [2, 7, 256])
+def test_fibonacci_directions(n):
+    """GetFibonacciSphere (VoxelMathHelper.h:48-71): unit vectors, heights
+    stepping evenly from +1 to -1, and successive azimuths one golden angle
+    pi * (3 - sqrt(5)) apart (mod 2 pi), i.e. pi * (sqrt(5) - 1) forward"""
+    d = O.pool_bake_directions(n).astype(np.float64)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-6)
+    assert np.allclose(d[:, 1], 1.0 - 2.0 * np.arange(n) / (n - 1), atol=1e-6)
+    inner = np.abs(d[:, 1]) < 0.999
+    az = np.arctan2(d[:, 2], d[:, 0])
+    step = np.mod(np.diff(az), 2 * np.pi)
+    both = inner[1:] & inner[:-1]
+    assert np.allclose(step[both], np.mod(np.pi * (np.sqrt(5) - 1), 2 * np.pi), atol=1e-4)
+
+
+def test_snap_is_the_nearest_baked_direction():
+    d = O.pool_bake_directions(256)
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        v = rng.normal(size=3)
+        f = O.pool_forward(v)
+        got = O.pool_snap(d, f)
+        dist = np.linalg.norm(d.astype(np.float64) - f.astype(np.float64), axis=1)
+        order = np.argsort(dist)
+        if dist[order[1]] - dist[order[0]] < 1e-6:
+            continue  # a near-tie between two directions
+        assert np.array_equal(got, d[order[0]])
+
+
+def test_baked_pool_equals_exact_pool_on_the_snapped_view():
+    sc, _ = _scene()
+    p = O.pool_params(forward_radius=4, backward_radius=2, view_angle=100.0, capacity=48,
+                      quads_per_slot=3000, max_loads=12, greedy=True)
+    baked = O.OraclePool(sc, p, bake_directions=256)
+    exact = O.OraclePool(sc, p)
+    dirs = O.pool_bake_directions(256)
+    for cam, fwd in [((1, 1, 1), (1, 0.3, 0)), ((3, 2, 2), (0.2, 0.1, 1)),
+                     ((5, 2, 5), (-1, 0.4, -0.5))]:
+        snapped = O.pool_snap(dirs, O.pool_forward(fwd))
+        assert baked.update(cam, fwd)[:3] == exact.update(cam, snapped)[:3]
+        assert np.array_equal(baked.lookup, exact.lookup)
+        assert np.array_equal(baked.slot_chunk, exact.slot_chunk)
+
+
 def _cube(R):
     r = range(-R, R + 1)
     return [(x, y, z) for z in r for y in r for x in r]
