@@ -1,5 +1,6 @@
 // C-ABI entry points: errors, context, buffers, streams, events, scenes,
 // raymarch.  Kernels live in raymarch.hip / mesh.hip / voxelize.hip.
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -102,6 +103,86 @@ __global__ void coarse_groups_kernel(const uint32_t *node_slots, uint64_t n_node
   atomicOr(&groups[i / 4096u], 1ull << ((bx >> 2) + 4u * (by >> 2) + 16u * (bz >> 2)));
 }
 
+// min / max brick coordinates of the allocated bricks: box[0..2] min,
+// box[3..5] max (one atomic per wave)
+__device__ __forceinline__ void box_reduce(bool live, uint32_t bx, uint32_t by, uint32_t bz,
+                                           int *box) {
+  int mn[3] = {live ? (int)bx : INT_MAX, live ? (int)by : INT_MAX, live ? (int)bz : INT_MAX};
+  int mx[3] = {live ? (int)bx : -1, live ? (int)by : -1, live ? (int)bz : -1};
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const int o1 = __shfl_xor(mn[a], off, 64), o2 = __shfl_xor(mx[a], off, 64);
+      mn[a] = o1 < mn[a] ? o1 : mn[a];
+      mx[a] = o2 > mx[a] ? o2 : mx[a];
+    }
+  if ((threadIdx.x & 63u) == 0 && mx[0] >= 0)
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      atomicMin(&box[a], mn[a]);
+      atomicMax(&box[3 + a], mx[a]);
+    }
+}
+
+__global__ void box_brickmap_kernel(const uint32_t *slots, uint32_t nb, int *box) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t cells = (uint64_t)nb * nb * nb;
+  const bool live = i < cells && slots[i] != kEmptySlot;
+  box_reduce(live, (uint32_t)(i % nb), (uint32_t)((i / nb) % nb), (uint32_t)(i / ((uint64_t)nb * nb)),
+             box);
+}
+
+__global__ void box_two_level_kernel(const uint32_t *top, uint32_t nt, const uint32_t *node_slots,
+                                     int *box) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t cells = (uint64_t)nt * nt * nt * 4096u;
+  bool live = false;
+  uint32_t bx = 0, by = 0, bz = 0;
+  if (i < cells) {
+    const uint64_t t = i / 4096u;
+    const uint32_t l = (uint32_t)(i % 4096u);
+    const uint32_t node = top[t];
+    live = node != kEmptySlot && node_slots[(uint64_t)node * 4096u + l] != kEmptySlot;
+    bx = (uint32_t)(t % nt) * 16u + (l & 15u);
+    by = (uint32_t)((t / nt) % nt) * 16u + ((l >> 4) & 15u);
+    bz = (uint32_t)(t / ((uint64_t)nt * nt)) * 16u + (l >> 8);
+  }
+  box_reduce(live, bx, by, bz, box);
+}
+
+// the box of the allocated bricks, in voxels (empty when there are none)
+static meso_status build_box(meso_scene *sc, hipStream_t stream) {
+  int init[6] = {INT_MAX, INT_MAX, INT_MAX, -1, -1, -1}, res[6];
+  int *d = nullptr;
+  MESO_HIP(hipMalloc((void **)&d, sizeof(init)));
+  hipError_t e = hipMemcpyAsync(d, init, sizeof(init), hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) {
+    if (sc->kind == MESO_SCENE_BRICKMAP_U16) {
+      const uint32_t nb = (uint32_t)sc->n / 8;
+      const uint64_t cells = (uint64_t)nb * nb * nb;
+      hipLaunchKernelGGL(box_brickmap_kernel, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0,
+                         stream, sc->slots, nb, d);
+    } else {
+      const uint32_t nt = (uint32_t)sc->n / 128;
+      const uint64_t cells = (uint64_t)nt * nt * nt * 4096u;
+      hipLaunchKernelGGL(box_two_level_kernel, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0,
+                         stream, sc->top, nt, sc->slots, d);
+    }
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(res, d, sizeof(res), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(MESO_ERR_RUNTIME, "scene box: %s", hipGetErrorString(e));
+  const bool empty = res[3] < 0;
+  for (int a = 0; a < 3; a++) {
+    sc->box_lo[a] = empty ? 0 : res[a] * 8;
+    sc->box_hi[a] = empty ? 0 : (res[3 + a] + 1) * 8;
+  }
+  return MESO_OK;
+}
+
 meso_status build_coarse(meso_scene *sc, hipStream_t stream) {
   if (sc->kind == MESO_SCENE_BRICKMAP_U16) {
     const uint64_t nb = (uint64_t)sc->n / 8;
@@ -132,6 +213,8 @@ meso_status build_coarse(meso_scene *sc, hipStream_t stream) {
     }
   }
   MESO_HIP(hipStreamSynchronize(stream));
+  if (sc->kind == MESO_SCENE_BRICKMAP_U16 || sc->kind == MESO_SCENE_TWO_LEVEL_U16)
+    return build_box(sc, stream);
   return MESO_OK;
 }
 

@@ -58,6 +58,9 @@ struct DevScene {
   const uint32_t *occ32;   // brick map: one bit per 32^3 block holding a brick
   const uint32_t *occ128;  // brick map: one bit per 128^3 block
   const uint64_t *groups;  // two-level: per node, one bit per 4^3-brick group
+  // the box [lo, hi) that holds every solid voxel: rays are clipped to it and
+  // walks end when they step out of it
+  int32_t lo[3], hi[3];
 };
 
 struct RayOut {
@@ -121,8 +124,22 @@ struct meso_scene {
   int64_t n_nodes = 0;
   uint64_t bytes = 0;
   int64_t solid = -1;
+  // voxels [box_lo, box_hi) hold every allocated brick (brick scenes; set by
+  // build_coarse); dense grids use the whole grid
+  int32_t box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};
   meso::DevScene dev() const {
-    return {n, dense, slots, masks, voxels, top, occ32, occ128, groups};
+    const bool dense_grid = kind == MESO_SCENE_DENSE_U8;
+    return {n,
+            dense,
+            slots,
+            masks,
+            voxels,
+            top,
+            occ32,
+            occ128,
+            groups,
+            {dense_grid ? 0 : box_lo[0], dense_grid ? 0 : box_lo[1], dense_grid ? 0 : box_lo[2]},
+            {dense_grid ? n : box_hi[0], dense_grid ? n : box_hi[1], dense_grid ? n : box_hi[2]}};
   }
 };
 

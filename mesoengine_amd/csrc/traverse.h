@@ -90,10 +90,11 @@ __device__ __forceinline__ uint32_t lin3u(uint32_t x, uint32_t y, uint32_t z, ui
 // scene and never moving against the ray; at sh = 0 the block is the cell
 // itself, so the clamp keeps them, which is the plain voxel step.  Written
 // without branches (selects only), so that the lanes of a wave stepping at
-// different levels stay converged.  Returns false when the ray leaves the
-// scene.
+// different levels stay converged.  Returns false when the step along the
+// chosen axis leaves the scene's solid box (nothing solid lies beyond it).
 template <bool LEVELS = true>
-__device__ __forceinline__ bool advance(const Ray &r, int32_t n, int32_t sh, WalkState &w) {
+__device__ __forceinline__ bool advance(const Ray &r, const DevScene &s, int32_t sh, WalkState &w) {
+  const int32_t n = s.n;
   float tb[3];
   int32_t bc[3], pl[3];
 #pragma unroll
@@ -111,7 +112,9 @@ __device__ __forceinline__ bool advance(const Ray &r, int32_t n, int32_t sh, Wal
   const int32_t pax = a0 ? pl[0] : (a1 ? pl[1] : pl[2]);
   const int32_t na = pax - (int32_t)((r.negmask >> ax) & 1u);
   const int32_t face = 4 - 2 * ax + (int32_t)((r.posmask >> ax) & 1u);  // entry_face
-  if (na < 0 || na >= n) return false;
+  const int32_t blo = a0 ? s.lo[0] : (a1 ? s.lo[1] : s.lo[2]);
+  const int32_t bhi = a0 ? s.hi[0] : (a1 ? s.hi[1] : s.hi[2]);
+  if (na < blo || na >= bhi) return false;
   if constexpr (!LEVELS) {  // voxel steps only (dense grid): the other cells stay
 #pragma unroll
     for (int b = 0; b < 3; b++) w.cell[b] = b == ax ? na : w.cell[b];
@@ -209,7 +212,7 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkSt
       material = v;
       return kWalkHit;
     }
-    return advance<false>(r, n, 0, w) ? kWalkOn : kWalkOut;
+    return advance<false>(r, s, 0, w) ? kWalkOn : kWalkOut;
   } else {
     const uint32_t n128 = (uint32_t)(n + 127) >> 7, n32 = (uint32_t)(n + 31) >> 5;
     // one flat pass over the levels (no nesting, so a wave whose lanes sit
@@ -278,7 +281,7 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkSt
     }
     const int32_t sh =
         c.e128 ? 7 : (c.e32 ? 5 : (!in_brick ? 3 : (e4 ? 2 : (e2 ? 1 : 0))));
-    return advance(r, n, sh, w) ? kWalkOn : kWalkOut;
+    return advance(r, s, sh, w) ? kWalkOn : kWalkOut;
   }
 }
 
@@ -330,23 +333,23 @@ struct Hit {
   uint16_t material;
 };
 
-// Set up the ray o + t d, clip it to [0,n]^3 and find its entry cell; false
-// when it misses the scene.
+// Set up the ray o + t d, clip it to the scene's solid box [lo, hi) and find
+// its entry cell; false when it misses the box.
 __device__ __forceinline__ bool trace_begin(const DevScene &s, const float o[3], const float d[3],
                                             Ray &r, WalkState &w) {
   ray_setup(r, o, d);
-  const int32_t n = s.n;
-  const float fn = (float)n;
   float t_enter = 0.0f, t_exit = __builtin_inff();
   int ea = -1;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
+    if (s.lo[a] >= s.hi[a]) return false;  // nothing solid
+    const float bl = (float)s.lo[a], bh = (float)s.hi[a];
     if (r.step[a] == 0) {
-      if (o[a] < 0.0f || o[a] >= fn) return false;
+      if (o[a] < bl || o[a] >= bh) return false;
       continue;
     }
-    float lo = plane_t(r.o[a], r.inv[a], r.step[a] > 0 ? 0.0f : fn);
-    float hi = plane_t(r.o[a], r.inv[a], r.step[a] > 0 ? fn : 0.0f);
+    float lo = plane_t(r.o[a], r.inv[a], r.step[a] > 0 ? bl : bh);
+    float hi = plane_t(r.o[a], r.inv[a], r.step[a] > 0 ? bh : bl);
     if (lo > t_enter) {
       t_enter = lo;
       ea = a;
@@ -357,13 +360,13 @@ __device__ __forceinline__ bool trace_begin(const DevScene &s, const float o[3],
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     if (a == ea) {
-      w.cell[a] = r.step[a] > 0 ? 0 : n - 1;
+      w.cell[a] = r.step[a] > 0 ? s.lo[a] : s.hi[a] - 1;
     } else {
       float dt = d[a] * t_enter;
       float p = o[a] + dt;
       int32_t c = (int32_t)__builtin_floorf(p);
-      c = c < 0 ? 0 : c;
-      c = c > n - 1 ? n - 1 : c;
+      c = c < s.lo[a] ? s.lo[a] : c;
+      c = c > s.hi[a] - 1 ? s.hi[a] - 1 : c;
       w.cell[a] = c;
     }
   }
@@ -400,11 +403,13 @@ __device__ __forceinline__ bool trace(const DevScene &s, const float o[3], const
   return true;
 }
 
-// true when a secondary ray can start in cell `start` (inside the scene);
-// outside it, the ray is unoccluded
+// true when a secondary ray can start in cell `start` (inside the scene's
+// solid box).  A start outside it is the neighbour across a face of a voxel
+// on the box's side, and the ray travels into that face's half-space, away
+// from the box, so it is unoccluded.
 __device__ __forceinline__ bool start_inside(const DevScene &s, const int32_t start[3]) {
-  return start[0] >= 0 && start[0] < s.n && start[1] >= 0 && start[1] < s.n && start[2] >= 0 &&
-         start[2] < s.n;
+  return start[0] >= s.lo[0] && start[0] < s.hi[0] && start[1] >= s.lo[1] && start[1] < s.hi[1] &&
+         start[2] >= s.lo[2] && start[2] < s.hi[2];
 }
 
 // A secondary ray o + t d that walks from cell `start` at t = 0.

@@ -37,7 +37,7 @@ class Scene(C.Structure):
     _fields_ = [("kind", C.c_int32), ("n", C.c_int32), ("dense", C.c_void_p),
                 ("slots", C.c_void_p), ("masks", C.c_void_p), ("voxels", C.c_void_p),
                 ("top", C.c_void_p), ("occ32", C.c_void_p), ("occ128", C.c_void_p),
-                ("groups", C.c_void_p)]
+                ("groups", C.c_void_p), ("box_lo", C.c_int32 * 3), ("box_hi", C.c_int32 * 3)]
 
 
 class Counters(C.Structure):
@@ -120,6 +120,7 @@ def lib():
             "mo_build_coarse": (None, [P, C.c_int64, P, P, P]),
             "mo_edit_chunks": (None, [P, P, P, P, C.c_int32, C.c_uint32]),
             "mo_scene_chunk": (None, [P, C.c_int32, C.c_int32, C.c_int32, P]),
+            "mo_scene_box": (None, [P, P, P]),
             "mo_pool_view_threshold": (C.c_float, [C.c_float]),
             "mo_pool_forward": (None, [P, P]),
             "mo_pool_bake_directions": (None, [C.c_int32, P]),
@@ -259,6 +260,11 @@ class OracleScene:
             self.groups = np.zeros(max(n_nodes, 1), np.uint64)
             self.s.groups = _p(self.groups)
             lib().mo_build_coarse(self.ptr, n_nodes, None, None, _p(self.groups))
+        lib().mo_scene_box(self.ptr, self.s.box_lo, self.s.box_hi)
+
+    @property
+    def box(self):
+        return tuple(self.s.box_lo), tuple(self.s.box_hi)
 
     @property
     def ptr(self):

@@ -524,8 +524,10 @@ static int32_t ceil_div(int32_t a, int32_t b) { return (a + b - 1) / b; }
  * nearest plane (ties x, then y).  For L > 1 the other two coordinates come
  * from the exit point, clamped into the block (and into the scene) and never
  * moving against the ray; for L = 1 they are unchanged, so the step is the
- * plain Amanatides & Woo voxel step.  Returns 0 when the ray leaves. */
-static int advance(const ray_t *r, int32_t n, int32_t L, walk_state *w) {
+ * plain Amanatides & Woo voxel step.  Returns 0 when the step along the
+ * chosen axis leaves the scene's box (nothing solid lies beyond it). */
+static int advance(const ray_t *r, int32_t n, const int32_t *box_lo, const int32_t *box_hi,
+                   int32_t L, walk_state *w) {
   int32_t bc[3];
   float tb[3];
   for (int a = 0; a < 3; a++) {
@@ -537,7 +539,7 @@ static int advance(const ray_t *r, int32_t n, int32_t L, walk_state *w) {
   int a = argmin3(tb);
   float t = tb[a];
   int32_t na = r->step[a] > 0 ? (bc[a] + 1) * L : bc[a] * L - 1;
-  if (na < 0 || na >= n) return 0;
+  if (na < box_lo[a] || na >= box_hi[a]) return 0;
   if (L > 1) {
     for (int b = 0; b < 3; b++) {
       if (b == a) continue;
@@ -658,7 +660,7 @@ static int walk(const mo_scene *s, const ray_t *r, walk_state *w, float tmax, ui
         }
       }
     }
-    if (!advance(r, n, L, w)) return 0;
+    if (!advance(r, n, s->box_lo, s->box_hi, L, w)) return 0;
   }
 }
 
@@ -695,20 +697,22 @@ int mo_trace(const mo_scene *s, const float o[3], const float d[3], mo_hit *hit,
   ray_t r;
   ray_setup(&r, o, d);
   const int32_t n = s->n;
-  const float fn = (float)n;
   hit->index = -1;
   hit->face = MO_FACE_MISS;
   hit->t = INFINITY;
   hit->material = 0;
+  /* clip to the scene's box */
   float t_enter = 0.0f, t_exit = INFINITY;
   int ea = -1;
   for (int a = 0; a < 3; a++) {
+    if (s->box_lo[a] >= s->box_hi[a]) return 0; /* nothing solid */
+    const float bl = (float)s->box_lo[a], bh = (float)s->box_hi[a];
     if (r.step[a] == 0) {
-      if (o[a] < 0.0f || o[a] >= fn) return 0;
+      if (o[a] < bl || o[a] >= bh) return 0;
       continue;
     }
-    float lo = plane_t(&r, a, r.step[a] > 0 ? 0.0f : fn);
-    float hi = plane_t(&r, a, r.step[a] > 0 ? fn : 0.0f);
+    float lo = plane_t(&r, a, r.step[a] > 0 ? bl : bh);
+    float hi = plane_t(&r, a, r.step[a] > 0 ? bh : bl);
     if (lo > t_enter) { t_enter = lo; ea = a; }
     if (hi < t_exit) t_exit = hi;
   }
@@ -716,13 +720,13 @@ int mo_trace(const mo_scene *s, const float o[3], const float d[3], mo_hit *hit,
   walk_state w;
   for (int a = 0; a < 3; a++) {
     if (a == ea) {
-      w.cell[a] = r.step[a] > 0 ? 0 : n - 1;
+      w.cell[a] = r.step[a] > 0 ? s->box_lo[a] : s->box_hi[a] - 1;
     } else {
       float dt = d[a] * t_enter;
       float p = o[a] + dt;
       int32_t c = (int32_t)floorf(p);
-      if (c < 0) c = 0;
-      if (c > n - 1) c = n - 1;
+      if (c < s->box_lo[a]) c = s->box_lo[a];
+      if (c > s->box_hi[a] - 1) c = s->box_hi[a] - 1;
       w.cell[a] = c;
     }
   }
@@ -741,9 +745,11 @@ int mo_trace(const mo_scene *s, const float o[3], const float d[3], mo_hit *hit,
 
 int mo_occluded(const mo_scene *s, const float o[3], const float d[3], const int32_t start[3],
                 float tmax, mo_counters *cnt) {
-  const int32_t n = s->n;
+  /* a start outside the box (the neighbour across a face of a voxel on the
+   * box's side) sees nothing: the ray travels into the face's half-space,
+   * away from the box */
   for (int a = 0; a < 3; a++)
-    if (start[a] < 0 || start[a] >= n) return 0;
+    if (start[a] < s->box_lo[a] || start[a] >= s->box_hi[a]) return 0;
   ray_t r;
   ray_setup(&r, o, d);
   walk_state w;
@@ -1200,6 +1206,32 @@ void mo_edit_chunks(const mo_scene *s, uint16_t *voxels, uint64_t *masks, const 
         masks[(int64_t)slot * 8 + lz] = m;
       }
     }
+  }
+}
+
+void mo_scene_box(const mo_scene *s, int32_t lo[3], int32_t hi[3]) {
+  if (s->kind == MO_SCENE_DENSE_U8) {
+    for (int a = 0; a < 3; a++) {
+      lo[a] = 0;
+      hi[a] = s->n;
+    }
+    return;
+  }
+  const int32_t nb = s->n / 8;
+  int32_t mn[3] = {nb, nb, nb}, mx[3] = {-1, -1, -1};
+  for (int32_t bz = 0; bz < nb; bz++)
+    for (int32_t by = 0; by < nb; by++)
+      for (int32_t bx = 0; bx < nb; bx++) {
+        if (brick_slot(s, bx, by, bz) == MO_EMPTY_SLOT) continue;
+        const int32_t b[3] = {bx, by, bz};
+        for (int a = 0; a < 3; a++) {
+          if (b[a] < mn[a]) mn[a] = b[a];
+          if (b[a] > mx[a]) mx[a] = b[a];
+        }
+      }
+  for (int a = 0; a < 3; a++) {
+    lo[a] = mx[a] < 0 ? 0 : mn[a] * 8;
+    hi[a] = mx[a] < 0 ? 0 : (mx[a] + 1) * 8;
   }
 }
 

@@ -155,6 +155,10 @@ typedef struct {
   const uint32_t *occ32;     /* brick map: ceil(n/32)^3 bits */
   const uint32_t *occ128;    /* brick map: ceil(n/128)^3 bits */
   const uint64_t *groups;    /* two-level: per node, bit of each 4^3-brick group */
+  /* the box [box_lo, box_hi) that holds every solid voxel (mo_scene_box):
+   * rays are clipped to it, walks end when they step out of it, and
+   * secondary rays that start outside it are unoccluded */
+  int32_t box_lo[3], box_hi[3];
 } mo_scene;
 
 /* Empty-space levels.  Brick map: occ32 / occ128 hold one bit per 32^3 /
@@ -165,6 +169,10 @@ typedef struct {
  * groups n_nodes words. */
 void mo_build_coarse(const mo_scene *s, int64_t n_nodes, uint32_t *occ32, uint32_t *occ128,
                      uint64_t *groups);
+/* The scene's box: brick map and two-level, the bricks that are allocated,
+ * in voxels (lo = 8 * min brick, hi = 8 * (max brick + 1)); empty when no
+ * brick is; dense grids, the whole grid. */
+void mo_scene_box(const mo_scene *s, int32_t lo[3], int32_t hi[3]);
 
 /* step counters for the algorithmic byte model */
 typedef struct {
