@@ -61,6 +61,9 @@ struct DevScene {
   // the box [lo, hi) that holds every solid voxel: rays are clipped to it and
   // walks end when they step out of it
   int32_t lo[3], hi[3];
+  // the same bounds as floats (exact: below 2^24), converted on the host so
+  // that they stay scalar kernel arguments (no scalar float convert on gfx950)
+  float flo[3], fhi[3];
 };
 
 struct RayOut {
@@ -129,17 +132,14 @@ struct meso_scene {
   int32_t box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};
   meso::DevScene dev() const {
     const bool dense_grid = kind == MESO_SCENE_DENSE_U8;
-    return {n,
-            dense,
-            slots,
-            masks,
-            voxels,
-            top,
-            occ32,
-            occ128,
-            groups,
-            {dense_grid ? 0 : box_lo[0], dense_grid ? 0 : box_lo[1], dense_grid ? 0 : box_lo[2]},
-            {dense_grid ? n : box_hi[0], dense_grid ? n : box_hi[1], dense_grid ? n : box_hi[2]}};
+    meso::DevScene d{n, dense, slots, masks, voxels, top, occ32, occ128, groups, {}, {}, {}, {}};
+    for (int a = 0; a < 3; a++) {
+      d.lo[a] = dense_grid ? 0 : box_lo[a];
+      d.hi[a] = dense_grid ? n : box_hi[a];
+      d.flo[a] = (float)d.lo[a];
+      d.fhi[a] = (float)d.hi[a];
+    }
+    return d;
   }
 };
 

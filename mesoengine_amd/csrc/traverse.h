@@ -112,9 +112,13 @@ __device__ __forceinline__ bool advance(const Ray &r, const DevScene &s, int32_t
   const int32_t pax = a0 ? pl[0] : (a1 ? pl[1] : pl[2]);
   const int32_t na = pax - (int32_t)((r.negmask >> ax) & 1u);
   const int32_t face = 4 - 2 * ax + (int32_t)((r.posmask >> ax) & 1u);  // entry_face
-  const int32_t blo = a0 ? s.lo[0] : (a1 ? s.lo[1] : s.lo[2]);
-  const int32_t bhi = a0 ? s.hi[0] : (a1 ? s.hi[1] : s.hi[2]);
-  if (na < blo || na >= bhi) return false;
+  // compared with each axis' bounds (scalar operands) and the chosen axis'
+  // answer kept: selecting the bounds per lane would hold them in vector
+  // registers for the whole walk
+  const bool out0 = na < s.lo[0] || na >= s.hi[0];
+  const bool out1 = na < s.lo[1] || na >= s.hi[1];
+  const bool out2 = na < s.lo[2] || na >= s.hi[2];
+  if (a0 ? out0 : (a1 ? out1 : out2)) return false;
   if constexpr (!LEVELS) {  // voxel steps only (dense grid): the other cells stay
 #pragma unroll
     for (int b = 0; b < 3; b++) w.cell[b] = b == ax ? na : w.cell[b];
@@ -343,7 +347,7 @@ __device__ __forceinline__ bool trace_begin(const DevScene &s, const float o[3],
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     if (s.lo[a] >= s.hi[a]) return false;  // nothing solid
-    const float bl = (float)s.lo[a], bh = (float)s.hi[a];
+    const float bl = s.flo[a], bh = s.fhi[a];
     if (r.step[a] == 0) {
       if (o[a] < bl || o[a] >= bh) return false;
       continue;
