@@ -80,10 +80,11 @@ def raymarch_leg(ctx, R, L, W, steps, warmup):
     # the same frames with their outputs copied to pinned host memory each frame
     pcie = pcie_inclusive(ctx, L, R, sc, cam, flags, sh, tg, w * h, max(3, steps // 2))
     rays = cnt["rays"] + cnt["shadow_rays"]
-    # bytes per frame: 4 B per brick slot visited, 64 B per non-empty brick
-    # mask fetched, 2 B material per hit, 13 B of outputs per pixel
-    alg_bytes = (4 * cnt["n_brickslot"] + 64 * cnt["n_brickmask"] + 2 * cnt["hits"]
-                 + 13 * cnt["rays"])
+    # bytes per frame: 4 B per brick slot visited, 8 B per non-empty brick
+    # entered (its 2^3-block word), 8 B per voxel-level mask word, 2 B
+    # material per hit, 13 B of outputs per pixel
+    alg_bytes = (4 * cnt["n_brickslot"] + 8 * cnt["n_brickmask"] + 8 * cnt["n_vox"]
+                 + 2 * cnt["hits"] + 13 * cnt["rays"])
     achieved = alg_bytes / (ms * 1e-3) / 1e9
     res = dict(ms=ms, wall_ms=wall * 1e3 / steps, rays=rays, primary=cnt["rays"],
                shadow=cnt["shadow_rays"], counts=cnt, alg_bytes=alg_bytes, achieved_gbs=achieved,

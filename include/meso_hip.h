@@ -206,7 +206,8 @@ meso_status meso_scene_update_bricks_async(meso_scene *scene, const uint32_t *d_
  * index)) every voxel of an allocated brick toggles (solid -> 0, empty -> 7)
  * and the brick masks follow.  Bricks are neither allocated nor freed.  The
  * chunk grid has ceil(n/32) chunks per axis (chunk index x + nc*(y + nc*z));
- * chunks outside it are skipped.
+ * chunks outside it are skipped.  The chunks of one call must be distinct
+ * (the edits of a repeated chunk race); a second call applies a repeat.
  * Replaces the per-chunk regeneration path FChunkManage::MultiThreadGenerator
  * (Runtimes/Voxel/Chunk/ChunkManager.h:160-210) for edited chunks. */
 meso_status meso_scene_edit_chunks(meso_scene *scene, const int32_t *d_chunk_xyz, int32_t n_chunks,
@@ -240,7 +241,10 @@ typedef struct {
 #define MESO_RAY_AO 2u
 #define MESO_RAY_COUNT 0x100u /* also accumulate step counters (slower) */
 
-/* Step counters (accumulated with atomics when MESO_RAY_COUNT is set). */
+/* Step counters (accumulated with atomics when MESO_RAY_COUNT is set).
+ * N_VOX: voxel reads of a dense grid, or voxel-level mask-word reads (8 B)
+ * of a brick scene; N_BRICKMASK: non-empty bricks entered (their 8-byte
+ * 2^3-block word read). */
 #define MESO_CNT_N_VOX 0
 #define MESO_CNT_N_TOP 1
 #define MESO_CNT_N_NODE 2

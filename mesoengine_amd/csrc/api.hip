@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "meso_internal.h"
+#include "traverse.h"
 
 namespace meso {
 
@@ -183,7 +184,24 @@ static meso_status build_box(meso_scene *sc, hipStream_t stream) {
   return MESO_OK;
 }
 
+// The 2^3-block bits of every brick, from its mask.
+__global__ void brick_blocks_kernel(const uint64_t *masks, uint64_t n_bricks, uint64_t *blocks) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_bricks) return;
+  blocks[i] = brick_blocks2(masks + i * 8u);
+}
+
 meso_status build_coarse(meso_scene *sc, hipStream_t stream) {
+  if (sc->kind == MESO_SCENE_BRICKMAP_U16 || sc->kind == MESO_SCENE_TWO_LEVEL_U16) {
+    const uint64_t nbr = (uint64_t)sc->n_bricks;
+    MESO_HIP(hipMalloc((void **)&sc->blocks, (nbr > 0 ? nbr : 1) * 8));
+    sc->bytes += nbr * 8;
+    if (nbr > 0) {
+      hipLaunchKernelGGL(brick_blocks_kernel, dim3((uint32_t)((nbr + 255) / 256)), dim3(256), 0,
+                         stream, sc->masks, nbr, sc->blocks);
+      MESO_HIP(hipGetLastError());
+    }
+  }
   if (sc->kind == MESO_SCENE_BRICKMAP_U16) {
     const uint64_t nb = (uint64_t)sc->n / 8;
     const uint64_t n32 = (nb + 3) / 4, n128 = (nb + 15) / 16;
@@ -220,12 +238,14 @@ meso_status build_coarse(meso_scene *sc, hipStream_t stream) {
 
 void free_scene_arrays(meso_scene *sc) {
   hipSetDevice(sc->ctx->device);
-  for (void *p : {(void *)sc->dense, (void *)sc->slots, (void *)sc->masks, (void *)sc->voxels,
-                  (void *)sc->top, (void *)sc->occ32, (void *)sc->occ128, (void *)sc->groups})
+  for (void *p : {(void *)sc->dense, (void *)sc->slots, (void *)sc->masks, (void *)sc->blocks,
+                  (void *)sc->voxels, (void *)sc->top, (void *)sc->occ32, (void *)sc->occ128,
+                  (void *)sc->groups})
     if (p) hipFree(p);
   sc->dense = nullptr;
   sc->slots = nullptr;
   sc->masks = nullptr;
+  sc->blocks = nullptr;
   sc->voxels = nullptr;
   sc->top = nullptr;
   sc->occ32 = sc->occ128 = nullptr;

@@ -53,6 +53,7 @@ struct DevScene {
   const uint8_t *dense;
   const uint32_t *slots;
   const uint64_t *masks;
+  const uint64_t *blocks;  // per brick slot: one bit per 2^3 block holding a solid voxel
   const uint16_t *voxels;
   const uint32_t *top;
   const uint32_t *occ32;   // brick map: one bit per 32^3 block holding a brick
@@ -118,6 +119,9 @@ struct meso_scene {
   uint8_t *dense = nullptr;
   uint32_t *slots = nullptr;
   uint64_t *masks = nullptr;
+  // per brick slot: bit cx + 4 cy + 16 cz set when the 2^3 block (cx,cy,cz)
+  // holds a solid voxel; written wherever a brick's mask is written
+  uint64_t *blocks = nullptr;
   uint16_t *voxels = nullptr;
   uint32_t *top = nullptr;
   uint32_t *occ32 = nullptr;
@@ -132,7 +136,8 @@ struct meso_scene {
   int32_t box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};
   meso::DevScene dev() const {
     const bool dense_grid = kind == MESO_SCENE_DENSE_U8;
-    meso::DevScene d{n, dense, slots, masks, voxels, top, occ32, occ128, groups, {}, {}, {}, {}};
+    meso::DevScene d{n,     dense, slots, masks, blocks, voxels, top, occ32, occ128, groups,
+                     {},    {},    {},    {}};
     for (int a = 0; a < 3; a++) {
       d.lo[a] = dense_grid ? 0 : box_lo[a];
       d.hi[a] = dense_grid ? n : box_hi[a];

@@ -146,30 +146,42 @@ __device__ __forceinline__ bool advance(const Ray &r, const DevScene &s, int32_t
   return true;
 }
 
+// The 4x4 2^2 blocks of an 8x8 plane of a brick (bit x + 8 y) that hold a set
+// bit, as 16 bits (bit cx + 4 cy): fold x and y pairs onto the even bits,
+// then gather the 16 even bits.
+__device__ __forceinline__ uint32_t plane_blocks2(uint64_t w) {
+  w |= w >> 1;
+  w |= w >> 8;
+  w &= 0x0055005500550055ull;
+  w = (w | (w >> 1)) & 0x0033003300330033ull;
+  w = (w | (w >> 2)) & 0x000F000F000F000Full;
+  w = (w | (w >> 12)) & 0x000000FF000000FFull;
+  w = (w | (w >> 24)) & 0xFFFFull;
+  return (uint32_t)w;
+}
+
 // One bit per aligned 2^3 block of a brick (bit cx + 4 cy + 16 cz), from its
-// 64-byte mask (word z, bit x + 8 y): OR the z pairs, fold x and y pairs
-// onto the even bits, then gather the 16 even bits of each pair plane.
+// 64-byte mask (word z, bit x + 8 y): the planes of each z pair ORed, folded.
 __device__ __forceinline__ uint64_t brick_blocks2(const uint64_t *mask) {
-  const uint4 *q = (const uint4 *)mask;  // 64-byte aligned
-  uint64_t m[8];
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const uint4 v = q[i];
-    m[2 * i] = (uint64_t)v.x | ((uint64_t)v.y << 32);
-    m[2 * i + 1] = (uint64_t)v.z | ((uint64_t)v.w << 32);
-  }
   uint64_t sub = 0;
 #pragma unroll
-  for (int p = 0; p < 4; p++) {
-    uint64_t w = m[2 * p] | m[2 * p + 1];
-    w |= w >> 1;
-    w |= w >> 8;
-    w &= 0x0055005500550055ull;
-    w = (w | (w >> 1)) & 0x0033003300330033ull;
-    w = (w | (w >> 2)) & 0x000F000F000F000Full;
-    w = (w | (w >> 12)) & 0x000000FF000000FFull;
-    w = (w | (w >> 24)) & 0xFFFFull;
-    sub |= w << (16 * p);
+  for (int p = 0; p < 4; p++)
+    sub |= (uint64_t)plane_blocks2(mask[2 * p] | mask[2 * p + 1]) << (16 * p);
+  return sub;
+}
+
+// A brick written by 512 lanes (lane = voxel x + 8 y + 64 z, each wave one z
+// plane) whose plane bits are `b` (the wave's ballot of solid voxels): the
+// brick's 2^3-block bits, returned to lane 0.  Every lane of the workgroup
+// must call it (it synchronises the workgroup).
+__device__ __forceinline__ uint64_t brick_blocks2_from_planes(uint64_t b) {
+  __shared__ uint32_t plane[8];
+  if ((threadIdx.x & 63u) == 0) plane[threadIdx.x >> 6] = plane_blocks2(b);
+  __syncthreads();
+  uint64_t sub = 0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int p = 0; p < 4; p++) sub |= (uint64_t)(plane[2 * p] | plane[2 * p + 1]) << (16 * p);
   }
   return sub;
 }
@@ -200,8 +212,8 @@ enum : int { kWalkOn = 0, kWalkHit = 1, kWalkOut = 2 };
 // else kWalkOn after stepping to the next cell.  KIND: 1 dense u8 (voxel
 // steps only), 2 brick map, 3 two-level.  A brick scene steps at the coarsest
 // empty level holding the cell: 128^3 block, 32^3 block, brick, the 4^3 and
-// 2^3 blocks of a non-empty brick (from its mask, read once per brick), else
-// the voxel.
+// 2^3 blocks of a non-empty brick (from its 2^3-block bits, read once per
+// brick), else the voxel (one 8-byte word of the brick's mask).
 template <int KIND, bool COUNT>
 __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkState &w,
                                          WalkCache &c, float tmax, uint16_t &material,
@@ -265,7 +277,7 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkSt
       }
       if (c.slot != kEmpty) {
         if (COUNT) cnt.n_brickmask++;
-        c.sub = brick_blocks2(s.masks + (uint64_t)c.slot * 8u);
+        c.sub = s.blocks[c.slot];
       }
     }
     // inside a non-empty brick: step over its empty 4^3 and 2^3 blocks
@@ -276,6 +288,7 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkSt
       e4 = ((c.sub >> (2u * (lx >> 2) + 8u * (ly >> 2) + 32u * (lz >> 2))) & 0x00330033ull) == 0;
       e2 = ((c.sub >> ((lx >> 1) + 4u * (ly >> 1) + 16u * (lz >> 1))) & 1ull) == 0;
       if (!e4 && !e2) {
+        if (COUNT) cnt.n_vox++;
         const uint64_t m = s.masks[(uint64_t)c.slot * 8u + lz];
         if ((m >> (lx + 8u * ly)) & 1ull) {
           material = s.voxels[(uint64_t)c.slot * 512u + lx + 8u * ly + 64u * lz];

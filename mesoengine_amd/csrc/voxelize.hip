@@ -7,6 +7,7 @@
 
 #include "meso_internal.h"
 #include "noise.h"
+#include "traverse.h"
 
 namespace meso {
 
@@ -344,8 +345,8 @@ __global__ void __launch_bounds__(256) sphere_chunk_kernel(const int32_t *chunk_
 __global__ void __launch_bounds__(512) brick_update_kernel(const uint32_t *edit_slots,
                                                            const uint16_t *edit_vox,
                                                            int64_t n_bricks, uint64_t *masks,
-                                                           uint16_t *voxels, uint32_t *bad,
-                                                           uint64_t n_edit) {
+                                                           uint64_t *blocks, uint16_t *voxels,
+                                                           uint32_t *bad, uint64_t n_edit) {
   const uint64_t e = block_id();
   if (e >= n_edit) return;
   const uint32_t slot = edit_slots[e];
@@ -357,6 +358,8 @@ __global__ void __launch_bounds__(512) brick_update_kernel(const uint32_t *edit_
   const unsigned long long b = __ballot(v != 0);
   voxels[(uint64_t)slot * 512u + threadIdx.x] = v;
   if ((threadIdx.x & 63u) == 0) masks[(uint64_t)slot * 8u + (threadIdx.x >> 6)] = (uint64_t)b;
+  const uint64_t sub = brick_blocks2_from_planes((uint64_t)b);
+  if (threadIdx.x == 0) blocks[slot] = sub;
 }
 
 // ------------------------------------------------------------------ chunk edits
@@ -377,8 +380,8 @@ __device__ __forceinline__ uint32_t scene_brick_slot(const DevScene &s, int32_t 
 // one workgroup of 512 lanes per (chunk, brick of the chunk); lane = voxel
 __global__ void __launch_bounds__(512) edit_chunks_kernel(DevScene s, int32_t kind,
                                                           const int32_t *xyz, uint32_t seed,
-                                                          uint64_t *masks, uint16_t *voxels,
-                                                          uint64_t n_blocks) {
+                                                          uint64_t *masks, uint64_t *blocks,
+                                                          uint16_t *voxels, uint64_t n_blocks) {
   const uint64_t bid = block_id();
   if (bid >= n_blocks) return;
   const uint64_t chunk = bid >> 6;
@@ -408,6 +411,8 @@ __global__ void __launch_bounds__(512) edit_chunks_kernel(DevScene s, int32_t ki
   }
   const unsigned long long m = __ballot(val != 0);
   if ((i & 63u) == 0) masks[(uint64_t)slot * 8u + (uint32_t)lz] = (uint64_t)m;
+  const uint64_t sub = brick_blocks2_from_planes((uint64_t)m);
+  if (i == 0) blocks[slot] = sub;
 }
 
 }  // namespace meso
@@ -647,7 +652,8 @@ meso_status meso_scene_edit_chunks(meso_scene *scene, const int32_t *d_chunk_xyz
   MESO_HIP(hipSetDevice(scene->ctx->device));
   const uint64_t n_blocks = (uint64_t)n_chunks * 64u;
   hipLaunchKernelGGL(edit_chunks_kernel, grid_1d(n_blocks), dim3(512), 0, s, scene->dev(),
-                     scene->kind, d_chunk_xyz, seed, scene->masks, scene->voxels, n_blocks);
+                     scene->kind, d_chunk_xyz, seed, scene->masks, scene->blocks, scene->voxels,
+                     n_blocks);
   MESO_HIP(hipGetLastError());
   return MESO_OK;
 }
@@ -666,7 +672,7 @@ meso_status meso_scene_update_bricks(meso_scene *scene, const uint32_t *d_slots,
   meso_status st = launch_zero(bad, 16, s);
   if (st != MESO_OK) return st;
   hipLaunchKernelGGL(brick_update_kernel, grid_1d((uint64_t)n_edit), dim3(512), 0, s, d_slots,
-                     d_voxels, scene->n_bricks, scene->masks, scene->voxels, bad,
+                     d_voxels, scene->n_bricks, scene->masks, scene->blocks, scene->voxels, bad,
                      (uint64_t)n_edit);
   MESO_HIP(hipGetLastError());
   uint32_t nbad = 0;
@@ -688,7 +694,7 @@ meso_status meso_scene_update_bricks_async(meso_scene *scene, const uint32_t *d_
   hipStream_t s = stream_of(scene->ctx, stream);
   MESO_HIP(hipSetDevice(scene->ctx->device));
   hipLaunchKernelGGL(brick_update_kernel, grid_1d((uint64_t)n_edit), dim3(512), 0, s, d_slots,
-                     d_voxels, scene->n_bricks, scene->masks, scene->voxels, d_skipped,
+                     d_voxels, scene->n_bricks, scene->masks, scene->blocks, scene->voxels, d_skipped,
                      (uint64_t)n_edit);
   MESO_HIP(hipGetLastError());
   return MESO_OK;

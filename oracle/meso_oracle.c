@@ -652,9 +652,12 @@ static int walk(const mo_scene *s, const ray_t *r, walk_state *w, float tmax, ui
               L = 4;
             } else if (brick_block_empty(mask, lx, ly, lz, 2)) {
               L = 2;
-            } else if ((mask[lz] >> (lx + 8 * ly)) & 1u) {
-              *material = s->voxels[(int64_t)slot * 512 + lx + 8 * ly + 64 * lz];
-              return 1;
+            } else {
+              cnt->n_vox++; /* one 8-byte mask word read */
+              if ((mask[lz] >> (lx + 8 * ly)) & 1u) {
+                *material = s->voxels[(int64_t)slot * 512 + lx + 8 * ly + 64 * lz];
+                return 1;
+              }
             }
           }
         }

@@ -174,7 +174,9 @@ void mo_build_coarse(const mo_scene *s, int64_t n_nodes, uint32_t *occ32, uint32
  * brick is; dense grids, the whole grid. */
 void mo_scene_box(const mo_scene *s, int32_t lo[3], int32_t hi[3]);
 
-/* step counters for the algorithmic byte model */
+/* step counters for the algorithmic byte model.  n_vox: voxel reads (dense,
+ * 1 byte) or voxel-level 8-byte mask-word reads (brick scenes); n_brickmask:
+ * bricks entered (their 8-byte 2^3-block word read) */
 typedef struct {
   int64_t n_vox, n_top, n_node, n_brickslot, n_brickmask, hits;
   int64_t rays, shadow_rays, ao_rays;

@@ -88,15 +88,19 @@ def test_pool_remesh_after_edits(ctx):
     pool.update((3, 3, 3), (0, 1, 0))
     opool.update((3, 3, 3), (0, 1, 0))
     _compare(pool, opool, "load")
-    # edit resident and non-resident chunks (plus repeats and one outside)
+    # edit resident and non-resident chunks and one outside, then repeat two
+    # of them in a second call (the chunks of one call are distinct)
     nc = opool.nc
     res = [int(c) for c in opool.slot_chunk if c != O.NO_SLOT]
     xyz = [[c % nc, (c // nc) % nc, c // (nc * nc)] for c in res[::3]]
-    xyz += [[0, 0, 7], [7, 7, 0], xyz[0], [-1, 0, 0]]
+    xyz += [[0, 0, 7], [7, 7, 0], [-1, 0, 0]]
     xyz = np.asarray(xyz, np.int32)
-    d = ctx.upload(xyz.ravel())
+    again = np.ascontiguousarray(xyz[[0, 1]])
+    d, d2 = ctx.upload(xyz.ravel()), ctx.upload(again.ravel())
     L.check(ctx.lib.meso_scene_edit_chunks(sc.h, d.ptr, len(xyz), 77, None))
+    L.check(ctx.lib.meso_scene_edit_chunks(sc.h, d2.ptr, len(again), 91, None))
     O.edit_chunks(osc, xyz, 77)
+    O.edit_chunks(osc, again, 91)
     pool.remesh(d, len(xyz))
     ctx.sync()
     q = kw["quads_per_slot"]
