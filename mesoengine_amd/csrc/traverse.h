@@ -80,8 +80,12 @@ struct StepCounts {
 __device__ __forceinline__ int64_t lin3(int64_t x, int64_t y, int64_t z, int64_t n) {
   return x + y * n + z * n * n;
 }
+// Index of (x,y,z) in an n^3 grid of blocks or bricks, with 24-bit multiplies
+// (full rate; a 32-bit multiply is quarter rate): scene edges are at most
+// 8192, so n <= 1024 and x, y, z < n, and every operand stays below 2^24 and
+// every product below 2^32.
 __device__ __forceinline__ uint32_t lin3u(uint32_t x, uint32_t y, uint32_t z, uint32_t n) {
-  return x + (y + z * n) * n;
+  return x + __umul24(y + __umul24(z, n), n);
 }
 
 // One step at level 2^sh (sh = 0 voxel, 3 brick, 5 and 7 empty blocks): leave
