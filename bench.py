@@ -80,14 +80,19 @@ def raymarch_leg(ctx, R, L, W, steps, warmup):
     # the same frames with their outputs copied to pinned host memory each frame
     pcie = pcie_inclusive(ctx, L, R, sc, cam, flags, sh, tg, w * h, max(3, steps // 2))
     rays = cnt["rays"] + cnt["shadow_rays"]
-    # bytes per frame: 4 B per brick slot visited, 8 B per non-empty brick
-    # entered (its 2^3-block word), 8 B per voxel-level mask word, 2 B
-    # material per hit, 13 B of outputs per pixel
-    alg_bytes = (4 * cnt["n_brickslot"] + 8 * cnt["n_brickmask"] + 8 * cnt["n_vox"]
-                 + 2 * cnt["hits"] + 13 * cnt["rays"])
+    # the path's per-ray byte model: 4 B per brick slot visited, 64 B per
+    # non-empty brick entered (its occupancy mask), 2 B material per hit,
+    # 13 B of outputs per pixel
+    alg_bytes = (4 * cnt["n_brickslot"] + 64 * cnt["n_brickmask"] + 2 * cnt["hits"]
+                 + 13 * cnt["rays"])
+    # what this layout reads instead of each 64 B mask: the brick's 8 B word
+    # of 2^3-block bits, plus one 8 B mask word per voxel-level test
+    layout_bytes = (4 * cnt["n_brickslot"] + 8 * cnt["n_brickmask"] + 8 * cnt["n_vox"]
+                    + 2 * cnt["hits"] + 13 * cnt["rays"])
     achieved = alg_bytes / (ms * 1e-3) / 1e9
     res = dict(ms=ms, wall_ms=wall * 1e3 / steps, rays=rays, primary=cnt["rays"],
                shadow=cnt["shadow_rays"], counts=cnt, alg_bytes=alg_bytes, achieved_gbs=achieved,
+               layout_bytes=layout_bytes,
                mrays=rays / (ms * 1e-3) / 1e6, mrays_primary=cnt["rays"] / (ms * 1e-3) / 1e6,
                tune=dict(variant=tune.variant, grid=tune.grid, tile=[tune.tile_w, tune.tile_h],
                          best_ms=round(tune.best_ms, 4), default_ms=round(tune.default_ms, 4)),
@@ -411,7 +416,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(rm["achieved_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(rm["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "alg_bytes_per_launch": rm["alg_bytes"]},
+                     "alg_bytes_per_launch": rm["alg_bytes"],
+                     "layout_bytes_per_launch": rm["layout_bytes"]},
         "cpu_baseline": cpu,
         "mrays_primary": round(rm["mrays_primary"], 2),
         "raymarch_tune": rm["tune"],

@@ -104,12 +104,50 @@ __global__ void coarse_groups_kernel(const uint32_t *node_slots, uint64_t n_node
   atomicOr(&groups[i / 4096u], 1ull << ((bx >> 2) + 4u * (by >> 2) + 16u * (bz >> 2)));
 }
 
+// Both brick kinds: per 32^3 block, one bit per allocated brick of its 4^3.
+__global__ void bricks32_kernel(const uint32_t *top, const uint32_t *slots, uint32_t nb,
+                                int32_t two_level, uint64_t cells, uint32_t n32,
+                                unsigned long long *bricks32) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cells) return;
+  uint32_t bx, by, bz;
+  if (two_level) {
+    const uint32_t nt = nb / 16u;
+    const uint64_t t = i / 4096u;
+    const uint32_t l = (uint32_t)(i % 4096u), node = top[t];
+    if (node == kEmptySlot || slots[(uint64_t)node * 4096u + l] == kEmptySlot) return;
+    bx = (uint32_t)(t % nt) * 16u + (l & 15u);
+    by = (uint32_t)((t / nt) % nt) * 16u + ((l >> 4) & 15u);
+    bz = (uint32_t)(t / ((uint64_t)nt * nt)) * 16u + (l >> 8);
+  } else {
+    if (slots[i] == kEmptySlot) return;
+    bx = (uint32_t)(i % nb);
+    by = (uint32_t)((i / nb) % nb);
+    bz = (uint32_t)(i / ((uint64_t)nb * nb));
+  }
+  const uint64_t w = (bx >> 2) + ((uint64_t)(by >> 2) + (uint64_t)(bz >> 2) * n32) * n32;
+  atomicOr(&bricks32[w], 1ull << ((bx & 3u) + 4u * (by & 3u) + 16u * (bz & 3u)));
+}
+
 // min / max brick coordinates of the allocated bricks: box[0..2] min,
 // box[3..5] max (one atomic per wave)
-__device__ __forceinline__ void box_reduce(bool live, uint32_t bx, uint32_t by, uint32_t bz,
-                                           int *box) {
-  int mn[3] = {live ? (int)bx : INT_MAX, live ? (int)by : INT_MAX, live ? (int)bz : INT_MAX};
-  int mx[3] = {live ? (int)bx : -1, live ? (int)by : -1, live ? (int)bz : -1};
+// The box of the cells a thread saw, in registers.
+struct BoxAcc {
+  int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {-1, -1, -1};
+  __device__ __forceinline__ void add(uint32_t bx, uint32_t by, uint32_t bz) {
+    const int c[3] = {(int)bx, (int)by, (int)bz};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      mn[a] = c[a] < mn[a] ? c[a] : mn[a];
+      mx[a] = c[a] > mx[a] ? c[a] : mx[a];
+    }
+  }
+};
+
+// Merges the boxes of a wave's lanes, then into `box` with one atomic per
+// bound (the kernels loop over their cells, so a wave merges once).
+__device__ __forceinline__ void box_reduce(BoxAcc acc, int *box) {
+  int *mn = acc.mn, *mx = acc.mx;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
@@ -127,29 +165,29 @@ __device__ __forceinline__ void box_reduce(bool live, uint32_t bx, uint32_t by, 
 }
 
 __global__ void box_brickmap_kernel(const uint32_t *slots, uint32_t nb, int *box) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t cells = (uint64_t)nb * nb * nb;
-  const bool live = i < cells && slots[i] != kEmptySlot;
-  box_reduce(live, (uint32_t)(i % nb), (uint32_t)((i / nb) % nb), (uint32_t)(i / ((uint64_t)nb * nb)),
-             box);
+  BoxAcc acc;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    if (slots[i] != kEmptySlot)
+      acc.add((uint32_t)(i % nb), (uint32_t)((i / nb) % nb), (uint32_t)(i / ((uint64_t)nb * nb)));
+  box_reduce(acc, box);
 }
 
 __global__ void box_two_level_kernel(const uint32_t *top, uint32_t nt, const uint32_t *node_slots,
                                      int *box) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t cells = (uint64_t)nt * nt * nt * 4096u;
-  bool live = false;
-  uint32_t bx = 0, by = 0, bz = 0;
-  if (i < cells) {
+  BoxAcc acc;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells;
+       i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t t = i / 4096u;
     const uint32_t l = (uint32_t)(i % 4096u);
     const uint32_t node = top[t];
-    live = node != kEmptySlot && node_slots[(uint64_t)node * 4096u + l] != kEmptySlot;
-    bx = (uint32_t)(t % nt) * 16u + (l & 15u);
-    by = (uint32_t)((t / nt) % nt) * 16u + ((l >> 4) & 15u);
-    bz = (uint32_t)(t / ((uint64_t)nt * nt)) * 16u + (l >> 8);
+    if (node == kEmptySlot || node_slots[(uint64_t)node * 4096u + l] == kEmptySlot) continue;
+    acc.add((uint32_t)(t % nt) * 16u + (l & 15u), (uint32_t)((t / nt) % nt) * 16u + ((l >> 4) & 15u),
+            (uint32_t)(t / ((uint64_t)nt * nt)) * 16u + (l >> 8));
   }
-  box_reduce(live, bx, by, bz, box);
+  box_reduce(acc, box);
 }
 
 // the box of the allocated bricks, in voxels (empty when there are none)
@@ -162,13 +200,13 @@ static meso_status build_box(meso_scene *sc, hipStream_t stream) {
     if (sc->kind == MESO_SCENE_BRICKMAP_U16) {
       const uint32_t nb = (uint32_t)sc->n / 8;
       const uint64_t cells = (uint64_t)nb * nb * nb;
-      hipLaunchKernelGGL(box_brickmap_kernel, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0,
-                         stream, sc->slots, nb, d);
+      hipLaunchKernelGGL(box_brickmap_kernel, dim3((uint32_t)std::min<uint64_t>((cells + 255) / 256, 1024)),
+                         dim3(256), 0, stream, sc->slots, nb, d);
     } else {
       const uint32_t nt = (uint32_t)sc->n / 128;
       const uint64_t cells = (uint64_t)nt * nt * nt * 4096u;
-      hipLaunchKernelGGL(box_two_level_kernel, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0,
-                         stream, sc->top, nt, sc->slots, d);
+      hipLaunchKernelGGL(box_two_level_kernel, dim3((uint32_t)std::min<uint64_t>((cells + 255) / 256, 1024)),
+                         dim3(256), 0, stream, sc->top, nt, sc->slots, d);
     }
     e = hipGetLastError();
   }
@@ -230,6 +268,21 @@ meso_status build_coarse(meso_scene *sc, hipStream_t stream) {
       MESO_HIP(hipGetLastError());
     }
   }
+  if (sc->kind == MESO_SCENE_BRICKMAP_U16 || sc->kind == MESO_SCENE_TWO_LEVEL_U16) {
+    const uint32_t nb = (uint32_t)sc->n / 8, n32 = (nb + 3) / 4;
+    const uint64_t words = (uint64_t)n32 * n32 * n32;
+    MESO_HIP(hipMalloc((void **)&sc->bricks32, words * 8));
+    sc->bytes += words * 8;
+    meso_status st = launch_zero(sc->bricks32, words * 8, stream);
+    if (st != MESO_OK) return st;
+    const bool two = sc->kind == MESO_SCENE_TWO_LEVEL_U16;
+    const uint64_t cells =
+        two ? (uint64_t)(nb / 16) * (nb / 16) * (nb / 16) * 4096u : (uint64_t)nb * nb * nb;
+    hipLaunchKernelGGL(bricks32_kernel, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0, stream,
+                       sc->top, sc->slots, nb, two ? 1 : 0, cells, n32,
+                       (unsigned long long *)sc->bricks32);
+    MESO_HIP(hipGetLastError());
+  }
   MESO_HIP(hipStreamSynchronize(stream));
   if (sc->kind == MESO_SCENE_BRICKMAP_U16 || sc->kind == MESO_SCENE_TWO_LEVEL_U16)
     return build_box(sc, stream);
@@ -240,7 +293,7 @@ void free_scene_arrays(meso_scene *sc) {
   hipSetDevice(sc->ctx->device);
   for (void *p : {(void *)sc->dense, (void *)sc->slots, (void *)sc->masks, (void *)sc->blocks,
                   (void *)sc->voxels, (void *)sc->top, (void *)sc->occ32, (void *)sc->occ128,
-                  (void *)sc->groups})
+                  (void *)sc->groups, (void *)sc->bricks32})
     if (p) hipFree(p);
   sc->dense = nullptr;
   sc->slots = nullptr;
@@ -250,6 +303,7 @@ void free_scene_arrays(meso_scene *sc) {
   sc->top = nullptr;
   sc->occ32 = sc->occ128 = nullptr;
   sc->groups = nullptr;
+  sc->bricks32 = nullptr;
 }
 
 // ------------------------------------------------------------------ RNG

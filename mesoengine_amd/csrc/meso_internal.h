@@ -59,6 +59,9 @@ struct DevScene {
   const uint32_t *occ32;   // brick map: one bit per 32^3 block holding a brick
   const uint32_t *occ128;  // brick map: one bit per 128^3 block
   const uint64_t *groups;  // two-level: per node, one bit per 4^3-brick group
+  // both brick kinds: per 32^3 block of the grid, bit (bx&3) + 4 (by&3) +
+  // 16 (bz&3) set when brick (bx,by,bz) of the block is allocated
+  const uint64_t *bricks32;
   // the box [lo, hi) that holds every solid voxel: rays are clipped to it and
   // walks end when they step out of it
   int32_t lo[3], hi[3];
@@ -127,6 +130,7 @@ struct meso_scene {
   uint32_t *occ32 = nullptr;
   uint32_t *occ128 = nullptr;
   uint64_t *groups = nullptr;
+  uint64_t *bricks32 = nullptr;  // see DevScene::bricks32
   int64_t n_bricks = 0;
   int64_t n_nodes = 0;
   uint64_t bytes = 0;
@@ -136,8 +140,8 @@ struct meso_scene {
   int32_t box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};
   meso::DevScene dev() const {
     const bool dense_grid = kind == MESO_SCENE_DENSE_U8;
-    meso::DevScene d{n,     dense, slots, masks, blocks, voxels, top, occ32, occ128, groups,
-                     {},    {},    {},    {}};
+    meso::DevScene d{n,      dense,  slots,  masks, blocks, voxels, top, occ32,
+                     occ128, groups, bricks32, {},  {},     {},     {}};
     for (int a = 0; a < 3; a++) {
       d.lo[a] = dense_grid ? 0 : box_lo[a];
       d.hi[a] = dense_grid ? n : box_hi[a];

@@ -200,12 +200,12 @@ struct WalkCache {
   int32_t c128, c32, cb;
   bool e128, e32;
   uint32_t node, slot;
-  uint64_t groups, sub;  // sub: 2^3-block bits of the current brick
+  uint64_t b32, sub;  // b32: allocated bricks of the 32^3 block; sub: 2^3-block bits of the brick
   __device__ __forceinline__ void reset() {
     c128 = c32 = cb = -1;
     e128 = e32 = false;
     node = slot = kEmpty;
-    groups = sub = 0;
+    b32 = sub = 0;
   }
 };
 
@@ -215,9 +215,10 @@ enum : int { kWalkOn = 0, kWalkHit = 1, kWalkOut = 2 };
 // kWalkOut when the entry t has reached tmax or the ray leaves the scene,
 // else kWalkOn after stepping to the next cell.  KIND: 1 dense u8 (voxel
 // steps only), 2 brick map, 3 two-level.  A brick scene steps at the coarsest
-// empty level holding the cell: 128^3 block, 32^3 block, brick, the 4^3 and
-// 2^3 blocks of a non-empty brick (from its 2^3-block bits, read once per
-// brick), else the voxel (one 8-byte word of the brick's mask).
+// empty level holding the cell: 128^3 block, 32^3 block (its word of
+// allocated bricks, which also names the empty bricks without reading their
+// slots), brick, the 4^3 and 2^3 blocks of a non-empty brick (from its 2^3-
+// block bits, read once per brick), else the voxel (one 8-byte mask word).
 template <int KIND, bool COUNT>
 __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkState &w,
                                          WalkCache &c, float tmax, uint16_t &material,
@@ -249,7 +250,6 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkSt
         if (COUNT) cnt.n_top++;
         c.node = s.top[lin3u(x >> 7, y >> 7, z >> 7, n128)];
         c.e128 = c.node == kEmpty;
-        c.groups = c.e128 ? 0ull : s.groups[c.node];
       } else {
         if (COUNT) cnt.n_coarse++;
         c.e128 = !occ_bit(s.occ128, lin3u(x >> 7, y >> 7, z >> 7, n128));
@@ -259,27 +259,27 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkSt
     if (!c.e128 && k32 != c.c32) {
       c.c32 = k32;
       c.cb = -1;
-      if constexpr (KIND == 3) {
-        c.e32 = !((c.groups >> (((x >> 5) & 3u) + 4u * ((y >> 5) & 3u) + 16u * ((z >> 5) & 3u))) &
-                  1ull);
-      } else {
-        if (COUNT) cnt.n_coarse++;
-        c.e32 = !occ_bit(s.occ32, lin3u(x >> 5, y >> 5, z >> 5, n32));
-      }
+      if (COUNT) cnt.n_coarse++;
+      c.b32 = s.bricks32[lin3u(x >> 5, y >> 5, z >> 5, n32)];
+      c.e32 = c.b32 == 0;
     }
     const bool coarse = c.e128 || c.e32;
     const int32_t kb = (int32_t)((x >> 3) | ((y >> 3) << 11) | ((z >> 3) << 22));
     if (!coarse && kb != c.cb) {
       c.cb = kb;
-      if constexpr (KIND == 3) {
-        if (COUNT) cnt.n_node++;
-        c.slot = s.slots[(uint64_t)c.node * 4096u +
-                         lin3u((x >> 3) & 15u, (y >> 3) & 15u, (z >> 3) & 15u, 16u)];
-      } else {
-        if (COUNT) cnt.n_brickslot++;
-        c.slot = s.slots[lin3u(x >> 3, y >> 3, z >> 3, (uint32_t)n >> 3)];
+      if (COUNT) {
+        if constexpr (KIND == 3)
+          cnt.n_node++;
+        else
+          cnt.n_brickslot++;
       }
-      if (c.slot != kEmpty) {
+      c.slot = kEmpty;
+      if ((c.b32 >> (((x >> 3) & 3u) + 4u * ((y >> 3) & 3u) + 16u * ((z >> 3) & 3u))) & 1ull) {
+        if constexpr (KIND == 3)
+          c.slot = s.slots[(uint64_t)c.node * 4096u +
+                           lin3u((x >> 3) & 15u, (y >> 3) & 15u, (z >> 3) & 15u, 16u)];
+        else
+          c.slot = s.slots[lin3u(x >> 3, y >> 3, z >> 3, (uint32_t)n >> 3)];
         if (COUNT) cnt.n_brickmask++;
         c.sub = s.blocks[c.slot];
       }

@@ -35,12 +35,12 @@ def dirty_chunks(n, frame, seed, frac=0.01):
 
 
 def frame_bytes(cnt, index_bytes=8):
-    """Algorithmic bytes of one rendered frame: 4 B per top-level and node
-    pointer read, 8 B per non-empty brick entered (its 2^3-block word), 8 B
-    per voxel-level mask word, 2 B material per hit, and per pixel the
+    """Algorithmic bytes of one rendered frame (the path's per-ray byte
+    model): 4 B per top-level and node pointer read, 64 B per non-empty brick
+    entered (its occupancy mask), 2 B material per hit, and per pixel the
     outputs (index, 1 B face, 4 B t, 4 B shade)."""
-    return (4 * cnt["n_top"] + 4 * cnt["n_node"] + 8 * cnt["n_brickmask"] + 8 * cnt["n_vox"]
-            + 2 * cnt["hits"] + (index_bytes + 9) * cnt["rays"])
+    return (4 * cnt["n_top"] + 4 * cnt["n_node"] + 64 * cnt["n_brickmask"] + 2 * cnt["hits"]
+            + (index_bytes + 9) * cnt["rays"])
 
 
 class _Frames:

@@ -622,6 +622,7 @@ static int walk(const mo_scene *s, const ray_t *r, walk_state *w, float tmax, ui
           memcpy(c32, g, sizeof(g));
           cb[0] = cb[1] = cb[2] = -1;
           if (two) {
+            cnt->n_coarse++; /* the 32^3 block's word of allocated bricks */
             e32 = !((groups >> ((g[0] & 3) + 4 * (g[1] & 3) + 16 * (g[2] & 3))) & 1u);
           } else {
             cnt->n_coarse++;

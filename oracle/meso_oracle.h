@@ -180,7 +180,8 @@ void mo_scene_box(const mo_scene *s, int32_t lo[3], int32_t hi[3]);
 typedef struct {
   int64_t n_vox, n_top, n_node, n_brickslot, n_brickmask, hits;
   int64_t rays, shadow_rays, ao_rays;
-  int64_t n_coarse;  /* occ32 / occ128 words read (brick map) */
+  int64_t n_coarse;  /* coarse words read: 128^3-block bits (brick map) and
+                        each 32^3 block's word of allocated bricks (both) */
   int64_t n_iter;    /* traversal steps at any level */
 } mo_counters;
 
