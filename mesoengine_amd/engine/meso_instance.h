@@ -180,7 +180,7 @@ class FHipVoxelInstance {
   void ApplyEdits();
 
   // chunk edits applied by the next UpdatePhysics: chunk coordinates
-  // (x, y, z triples) and the seed of their sphere toggles
+  // (distinct x, y, z triples) and the seed of their sphere toggles
   void QueueChunkEdits(const std::vector<int32_t> &xyz, uint32_t seed);
 
   meso_status Status() const { return status_; }
