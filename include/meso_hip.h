@@ -153,7 +153,9 @@ typedef struct {
   int64_t solid_voxels; /* counted at generation; -1 if unknown */
 } meso_scene_info;
 
-/* host arrays are copied into library-owned device memory */
+/* host arrays are copied into library-owned device memory.  Brick scenes:
+ * every non-empty cell names a brick below n_bricks, and no brick is named
+ * by two cells (MESO_ERR_ARGUMENT_OUT_OF_RANGE otherwise). */
 meso_status meso_scene_create_dense_u8(meso_context *ctx, int32_t n, const uint8_t *voxels,
                                        meso_scene **out);
 meso_status meso_scene_create_brickmap_u16(meso_context *ctx, int32_t n, const uint32_t *slots,

@@ -222,21 +222,32 @@ static meso_status build_box(meso_scene *sc, hipStream_t stream) {
   return MESO_OK;
 }
 
-// The 2^3-block bits of every brick, from its mask.
-__global__ void brick_blocks_kernel(const uint64_t *masks, uint64_t n_bricks, uint64_t *blocks) {
+// Per brick cell (index i into slots): the 2^3-block bits of its brick, from
+// the brick's mask (0 for an empty cell), and 1 + the brick's cell.
+__global__ void brick_blocks_kernel(const uint32_t *slots, uint64_t cells, const uint64_t *masks,
+                                    uint64_t *blocks, uint32_t *slot_cell) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_bricks) return;
-  blocks[i] = brick_blocks2(masks + i * 8u);
+  if (i >= cells) return;
+  const uint32_t slot = slots[i];
+  blocks[i] = slot == kEmptySlot ? 0ull : brick_blocks2(masks + (uint64_t)slot * 8u);
+  if (slot != kEmptySlot) slot_cell[slot] = (uint32_t)i + 1u;
 }
 
 meso_status build_coarse(meso_scene *sc, hipStream_t stream) {
   if (sc->kind == MESO_SCENE_BRICKMAP_U16 || sc->kind == MESO_SCENE_TWO_LEVEL_U16) {
+    // cells = entries of slots (below 2^30: edges are at most 8192)
+    const uint64_t nb = (uint64_t)sc->n / 8;
+    const uint64_t cells = sc->kind == MESO_SCENE_BRICKMAP_U16 ? nb * nb * nb
+                                                               : (uint64_t)sc->n_nodes * 4096u;
     const uint64_t nbr = (uint64_t)sc->n_bricks;
-    MESO_HIP(hipMalloc((void **)&sc->blocks, (nbr > 0 ? nbr : 1) * 8));
-    sc->bytes += nbr * 8;
-    if (nbr > 0) {
-      hipLaunchKernelGGL(brick_blocks_kernel, dim3((uint32_t)((nbr + 255) / 256)), dim3(256), 0,
-                         stream, sc->masks, nbr, sc->blocks);
+    MESO_HIP(hipMalloc((void **)&sc->blocks, (cells > 0 ? cells : 1) * 8));
+    MESO_HIP(hipMalloc((void **)&sc->slot_cell, (nbr > 0 ? nbr : 1) * 4));
+    sc->bytes += cells * 8 + nbr * 4;
+    meso_status st = launch_zero(sc->slot_cell, (nbr > 0 ? nbr : 1) * 4, stream);
+    if (st != MESO_OK) return st;
+    if (cells > 0) {
+      hipLaunchKernelGGL(brick_blocks_kernel, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0,
+                         stream, sc->slots, cells, sc->masks, sc->blocks, sc->slot_cell);
       MESO_HIP(hipGetLastError());
     }
   }
@@ -292,13 +303,14 @@ meso_status build_coarse(meso_scene *sc, hipStream_t stream) {
 void free_scene_arrays(meso_scene *sc) {
   hipSetDevice(sc->ctx->device);
   for (void *p : {(void *)sc->dense, (void *)sc->slots, (void *)sc->masks, (void *)sc->blocks,
-                  (void *)sc->voxels, (void *)sc->top, (void *)sc->occ32, (void *)sc->occ128,
+                  (void *)sc->slot_cell, (void *)sc->voxels, (void *)sc->top, (void *)sc->occ32, (void *)sc->occ128,
                   (void *)sc->groups, (void *)sc->bricks32})
     if (p) hipFree(p);
   sc->dense = nullptr;
   sc->slots = nullptr;
   sc->masks = nullptr;
   sc->blocks = nullptr;
+  sc->slot_cell = nullptr;
   sc->voxels = nullptr;
   sc->top = nullptr;
   sc->occ32 = sc->occ128 = nullptr;
@@ -694,11 +706,19 @@ meso_status meso_scene_create_brickmap_u16(meso_context *ctx, int32_t n, const u
   MESO_CHECK(n_bricks == 0 || (masks && voxels), MESO_ERR_ARGUMENT_OUT_OF_RANGE,
              "masks/voxels NULL with %lld bricks", (long long)n_bricks);
   const int64_t nb = n / 8, ncell = nb * nb * nb;
-  // every slot must name an existing brick: the kernels index the pools with it
-  for (int64_t i = 0; i < ncell; i++)
+  // every slot must name an existing brick (the kernels index the pools with
+  // it), and no brick may be held by two cells (a brick's derived words are
+  // kept per cell)
+  std::vector<bool> held((size_t)n_bricks);
+  for (int64_t i = 0; i < ncell; i++) {
     MESO_CHECK(slots[i] == MESO_EMPTY_SLOT || (int64_t)slots[i] < n_bricks,
                MESO_ERR_ARGUMENT_OUT_OF_RANGE, "slot[%lld] = %u >= n_bricks %lld", (long long)i,
                slots[i], (long long)n_bricks);
+    if (slots[i] == MESO_EMPTY_SLOT) continue;
+    MESO_CHECK(!held[slots[i]], MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+               "brick %u is held by two cells (slot[%lld])", slots[i], (long long)i);
+    held[slots[i]] = true;
+  }
   MESO_HIP(hipSetDevice(ctx->device));
   auto *sc = new meso_scene();
   sc->ctx = ctx;
@@ -729,10 +749,16 @@ meso_status meso_scene_create_two_level_u16(meso_context *ctx, int32_t n, const 
   for (int64_t i = 0; i < ntop; i++)
     MESO_CHECK(top[i] == MESO_EMPTY_SLOT || (int64_t)top[i] < n_nodes,
                MESO_ERR_ARGUMENT_OUT_OF_RANGE, "top[%lld] = %u >= n_nodes", (long long)i, top[i]);
-  for (int64_t i = 0; i < n_nodes * 4096; i++)
+  std::vector<bool> held((size_t)n_bricks);
+  for (int64_t i = 0; i < n_nodes * 4096; i++) {
     MESO_CHECK(node_slots[i] == MESO_EMPTY_SLOT || (int64_t)node_slots[i] < n_bricks,
                MESO_ERR_ARGUMENT_OUT_OF_RANGE, "node slot %lld = %u >= n_bricks", (long long)i,
                node_slots[i]);
+    if (node_slots[i] == MESO_EMPTY_SLOT) continue;
+    MESO_CHECK(!held[node_slots[i]], MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+               "brick %u is held by two node cells (%lld)", node_slots[i], (long long)i);
+    held[node_slots[i]] = true;
+  }
   MESO_HIP(hipSetDevice(ctx->device));
   auto *sc = new meso_scene();
   sc->ctx = ctx;

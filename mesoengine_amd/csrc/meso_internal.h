@@ -53,7 +53,9 @@ struct DevScene {
   const uint8_t *dense;
   const uint32_t *slots;
   const uint64_t *masks;
-  const uint64_t *blocks;  // per brick slot: one bit per 2^3 block holding a solid voxel
+  // per brick cell (indexed like slots): one bit per 2^3 block of the brick
+  // holding a solid voxel (0 for an empty cell)
+  const uint64_t *blocks;
   const uint16_t *voxels;
   const uint32_t *top;
   const uint32_t *occ32;   // brick map: one bit per 32^3 block holding a brick
@@ -122,9 +124,12 @@ struct meso_scene {
   uint8_t *dense = nullptr;
   uint32_t *slots = nullptr;
   uint64_t *masks = nullptr;
-  // per brick slot: bit cx + 4 cy + 16 cz set when the 2^3 block (cx,cy,cz)
-  // holds a solid voxel; written wherever a brick's mask is written
+  // per brick cell (indexed like slots): bit cx + 4 cy + 16 cz set when the
+  // 2^3 block (cx,cy,cz) of the cell's brick holds a solid voxel; written
+  // wherever a brick's mask is written.  slot_cell[slot]: 1 + the cell (index
+  // into slots) that holds the brick, 0 when no cell does.
   uint64_t *blocks = nullptr;
+  uint32_t *slot_cell = nullptr;
   uint16_t *voxels = nullptr;
   uint32_t *top = nullptr;
   uint32_t *occ32 = nullptr;

@@ -275,13 +275,14 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkSt
       }
       c.slot = kEmpty;
       if ((c.b32 >> (((x >> 3) & 3u) + 4u * ((y >> 3) & 3u) + 16u * ((z >> 3) & 3u))) & 1ull) {
-        if constexpr (KIND == 3)
-          c.slot = s.slots[(uint64_t)c.node * 4096u +
-                           lin3u((x >> 3) & 15u, (y >> 3) & 15u, (z >> 3) & 15u, 16u)];
-        else
-          c.slot = s.slots[lin3u(x >> 3, y >> 3, z >> 3, (uint32_t)n >> 3)];
+        // the slot and the 2^3-block word are both kept per brick cell, so
+        // the two loads go out together (cells are below 2^30)
+        const uint32_t cell =
+            KIND == 3 ? c.node * 4096u + lin3u((x >> 3) & 15u, (y >> 3) & 15u, (z >> 3) & 15u, 16u)
+                      : lin3u(x >> 3, y >> 3, z >> 3, (uint32_t)n >> 3);
+        c.slot = s.slots[cell];
+        c.sub = s.blocks[cell];
         if (COUNT) cnt.n_brickmask++;
-        c.sub = s.blocks[c.slot];
       }
     }
     // inside a non-empty brick: step over its empty 4^3 and 2^3 blocks

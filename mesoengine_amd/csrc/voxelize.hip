@@ -345,7 +345,9 @@ __global__ void __launch_bounds__(256) sphere_chunk_kernel(const int32_t *chunk_
 __global__ void __launch_bounds__(512) brick_update_kernel(const uint32_t *edit_slots,
                                                            const uint16_t *edit_vox,
                                                            int64_t n_bricks, uint64_t *masks,
-                                                           uint64_t *blocks, uint16_t *voxels,
+                                                           uint64_t *blocks,
+                                                           const uint32_t *slot_cell,
+                                                           uint16_t *voxels,
                                                            uint32_t *bad, uint64_t n_edit) {
   const uint64_t e = block_id();
   if (e >= n_edit) return;
@@ -359,22 +361,23 @@ __global__ void __launch_bounds__(512) brick_update_kernel(const uint32_t *edit_
   voxels[(uint64_t)slot * 512u + threadIdx.x] = v;
   if ((threadIdx.x & 63u) == 0) masks[(uint64_t)slot * 8u + (threadIdx.x >> 6)] = (uint64_t)b;
   const uint64_t sub = brick_blocks2_from_planes((uint64_t)b);
-  if (threadIdx.x == 0) blocks[slot] = sub;
+  if (threadIdx.x == 0 && slot_cell[slot] != 0u) blocks[slot_cell[slot] - 1u] = sub;
 }
 
 // ------------------------------------------------------------------ chunk edits
-// slot of brick (bx,by,bz) in a brick map or two-level scene, or kEmptySlot
-__device__ __forceinline__ uint32_t scene_brick_slot(const DevScene &s, int32_t kind, int32_t bx,
+// cell (index into slots) of brick (bx,by,bz) in a brick map or two-level
+// scene, or kEmptySlot outside the grid or in an empty node
+__device__ __forceinline__ uint32_t scene_brick_cell(const DevScene &s, int32_t kind, int32_t bx,
                                                      int32_t by, int32_t bz) {
   const int32_t nb = s.n >> 3;
   if (bx < 0 || by < 0 || bz < 0 || bx >= nb || by >= nb || bz >= nb) return kEmptySlot;
   if (kind == MESO_SCENE_BRICKMAP_U16)
-    return s.slots[(uint32_t)bx + ((uint32_t)by + (uint32_t)bz * (uint32_t)nb) * (uint32_t)nb];
+    return (uint32_t)bx + ((uint32_t)by + (uint32_t)bz * (uint32_t)nb) * (uint32_t)nb;
   const uint32_t nt = (uint32_t)s.n >> 7;
   const uint32_t node = s.top[((uint32_t)bx >> 4) + (((uint32_t)by >> 4) + ((uint32_t)bz >> 4) * nt) * nt];
   if (node == kEmptySlot) return kEmptySlot;
-  return s.slots[(uint64_t)node * 4096u + ((uint32_t)bx & 15u) + 16u * ((uint32_t)by & 15u) +
-                 256u * ((uint32_t)bz & 15u)];
+  return node * 4096u + ((uint32_t)bx & 15u) + 16u * ((uint32_t)by & 15u) +
+         256u * ((uint32_t)bz & 15u);
 }
 
 // one workgroup of 512 lanes per (chunk, brick of the chunk); lane = voxel
@@ -391,7 +394,9 @@ __global__ void __launch_bounds__(512) edit_chunks_kernel(DevScene s, int32_t ki
   if (cx < 0 || cy < 0 || cz < 0 || cx >= nc || cy >= nc || cz >= nc) return;
   const int32_t bx = cx * 4 + (int32_t)(b & 3u), by = cy * 4 + (int32_t)((b >> 2) & 3u),
                 bz = cz * 4 + (int32_t)(b >> 4);
-  const uint32_t slot = scene_brick_slot(s, kind, bx, by, bz);
+  const uint32_t cell = scene_brick_cell(s, kind, bx, by, bz);
+  if (cell == kEmptySlot) return;  // uniform over the workgroup
+  const uint32_t slot = s.slots[cell];
   if (slot == kEmptySlot) return;  // uniform over the workgroup
   const uint32_t h0 = nhash32(seed ^ nhash32((uint32_t)cx + ((uint32_t)cy + (uint32_t)cz * (uint32_t)nc) * (uint32_t)nc));
   const uint32_t h1 = nhash32(h0), h2 = nhash32(h1), h3 = nhash32(h2);
@@ -412,7 +417,7 @@ __global__ void __launch_bounds__(512) edit_chunks_kernel(DevScene s, int32_t ki
   const unsigned long long m = __ballot(val != 0);
   if ((i & 63u) == 0) masks[(uint64_t)slot * 8u + (uint32_t)lz] = (uint64_t)m;
   const uint64_t sub = brick_blocks2_from_planes((uint64_t)m);
-  if (i == 0) blocks[slot] = sub;
+  if (i == 0) blocks[cell] = sub;
 }
 
 }  // namespace meso
@@ -672,7 +677,8 @@ meso_status meso_scene_update_bricks(meso_scene *scene, const uint32_t *d_slots,
   meso_status st = launch_zero(bad, 16, s);
   if (st != MESO_OK) return st;
   hipLaunchKernelGGL(brick_update_kernel, grid_1d((uint64_t)n_edit), dim3(512), 0, s, d_slots,
-                     d_voxels, scene->n_bricks, scene->masks, scene->blocks, scene->voxels, bad,
+                     d_voxels, scene->n_bricks, scene->masks, scene->blocks, scene->slot_cell,
+                     scene->voxels, bad,
                      (uint64_t)n_edit);
   MESO_HIP(hipGetLastError());
   uint32_t nbad = 0;
@@ -694,7 +700,8 @@ meso_status meso_scene_update_bricks_async(meso_scene *scene, const uint32_t *d_
   hipStream_t s = stream_of(scene->ctx, stream);
   MESO_HIP(hipSetDevice(scene->ctx->device));
   hipLaunchKernelGGL(brick_update_kernel, grid_1d((uint64_t)n_edit), dim3(512), 0, s, d_slots,
-                     d_voxels, scene->n_bricks, scene->masks, scene->blocks, scene->voxels, d_skipped,
+                     d_voxels, scene->n_bricks, scene->masks, scene->blocks, scene->slot_cell,
+                     scene->voxels, d_skipped,
                      (uint64_t)n_edit);
   MESO_HIP(hipGetLastError());
   return MESO_OK;

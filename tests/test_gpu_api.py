@@ -195,6 +195,51 @@ def test_update_bricks(ctx):
     sc.destroy()
 
 
+def test_host_bricks_unreferenced_and_held_twice(ctx):
+    """A brick no cell holds can be updated (it stays invisible); a brick
+    named by two cells is refused at creation."""
+    n = 128
+    g = ctx.scene_generate(L.GEN_TERRAIN_CAVES, L.SCENE_BRICKMAP_U16, n, 60)
+    a = g.arrays()
+    g.destroy()
+    nb = len(a["masks"])
+    rng = np.random.default_rng(9)
+    spare_v = (rng.integers(1, 4, 512) * (rng.random(512) < 0.5)).astype(np.uint16)
+    spare_m = np.zeros(8, np.uint64)
+    for i in np.nonzero(spare_v)[0]:
+        spare_m[i >> 6] |= np.uint64(1) << np.uint64(i & 63)
+    masks = np.concatenate([a["masks"], spare_m[None]])
+    vox = np.concatenate([a["voxels"], spare_v[None]])
+    sc = ctx.scene_brickmap(n, a["slots"], masks, vox)  # brick nb: held by no cell
+    edits = np.stack([np.full(512, 3, np.uint16), np.zeros(512, np.uint16)])
+    sel = np.array([nb, int(a["slots"][a["slots"] != 0xFFFFFFFF][7])], np.uint32)
+    ds, dv = ctx.upload(sel), ctx.upload(edits.ravel())
+    L.check(ctx.lib.meso_scene_update_bricks(sc.h, ds.ptr, dv.ptr, 2, None))
+    b = sc.arrays()
+    assert np.array_equal(b["voxels"][sel], edits)
+    cp = dict(eye=[-30.0, 120.0, -20.0], target=[64.0, 20.0, 64.0], up=[0.0, 1.0, 0.0])
+    cam = R.camera(cp["eye"], cp["target"], cp["up"], 60.0, 160, 100)
+    tg = R.RayTargets(ctx, 160, 100)
+    R.raymarch(ctx, sc, cam, L.RAY_SHADOW, R.shading(W.LIGHT), tg)
+    ctx.sync()
+    ocam = O.camera(cp["eye"], cp["target"], cp["up"], 60.0, 160, 100)
+    ref = O.raymarch(O.brickmap_scene(n, b["slots"], b["masks"], b["voxels"]), ocam, O.RAY_SHADOW,
+                     O.shading(W.LIGHT))
+    for x, y in zip(tg.results(), ref[:4]):
+        assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
+    sc.destroy()
+    twice = a["slots"].copy()
+    held = np.nonzero(twice != 0xFFFFFFFF)[0]
+    empty = np.nonzero(twice == 0xFFFFFFFF)[0]
+    twice[empty[0]] = twice[held[0]]
+    h = C.c_void_p()
+    m = np.ascontiguousarray(a["masks"])
+    v = np.ascontiguousarray(a["voxels"])
+    assert ctx.lib.meso_scene_create_brickmap_u16(ctx.h, n, twice.ctypes.data, nb, m.ctypes.data,
+                                                  v.ctypes.data, C.byref(h)) == L.ERR_ARG
+    assert b"two cells" in ctx.lib.meso_last_error_message()
+
+
 def test_argument_errors(ctx):
     lib = ctx.lib
     sc = ctx.scene_generate(L.GEN_SPHERES, L.SCENE_DENSE_U8, 64, 3, 4)
