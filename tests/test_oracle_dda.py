@@ -237,6 +237,28 @@ def test_axis_parallel_rays_distance():
     assert h.index == -1 and h.face == 0xFF and np.isinf(h.t) and cnt.n_vox == n
 
 
+def test_brick_walk_counts_by_hand():
+    """One brick holding one solid voxel (3,3,3); a ray along +x at y = z = 3.5.
+    The walk enters the brick (one slot read, one block word), steps over
+    the empty 2^3 block of x 0..1 at once, then tests voxels x = 2 and x = 3
+    at the voxel level (two mask-word reads) and hits x = 3."""
+    n = 16
+    slots = np.full(8, 0xFFFFFFFF, np.uint32)
+    slots[0] = 0
+    masks = np.zeros((1, 8), np.uint64)
+    masks[0, 3] = np.uint64(1) << np.uint64(3 + 8 * 3)
+    vox = np.zeros((1, 512), np.uint16)
+    vox[0, 3 + 8 * 3 + 64 * 3] = 5
+    sc = O.brickmap_scene(n, slots, masks, vox)
+    h, cnt = O.trace(sc, np.array([-4.0, 3.5, 3.5], np.float32), np.array([1, 0, 0], np.float32))
+    assert h.index == 3 + 16 * 3 + 256 * 3 and h.face == 5 and h.t == 7.0 and h.material == 5
+    assert (cnt.n_brickslot, cnt.n_brickmask, cnt.n_vox, cnt.n_iter) == (1, 1, 2, 3)
+    # the same ray one row higher (y = 4.5) stays in brick 0's 4^3 block of
+    # y 4..7, which is empty: one step across the brick, no voxel test
+    h, cnt = O.trace(sc, np.array([-4.0, 4.5, 3.5], np.float32), np.array([1, 0, 0], np.float32))
+    assert h.index == -1 and (cnt.n_brickmask, cnt.n_vox) == (1, 0)
+
+
 def test_raymarch_invariants_sphere_grid():
     g = O.spheres_dense(1, 128, 8)
     sc = O.dense_scene(g)
