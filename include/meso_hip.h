@@ -109,6 +109,10 @@ meso_status meso_stream_sync(meso_context *ctx, meso_stream stream);
 meso_status meso_event_create(meso_context *ctx, meso_event **out);
 meso_status meso_event_destroy(meso_event *ev);
 meso_status meso_event_record(meso_event *ev, meso_stream stream);
+/* Work queued on `stream` after this call starts after `ev` completes: the
+ * dependency a submit can name (ICommandBuffer::cmdDispatchThreadGroups'
+ * Dependencies, LVK.h:730-732), between two of this library's streams. */
+meso_status meso_stream_wait_event(meso_context *ctx, meso_stream stream, meso_event *ev);
 meso_status meso_event_sync(meso_event *ev);
 meso_status meso_event_elapsed_ms(meso_event *start, meso_event *end, float *ms);
 /* copies between device pointers and host memory, on a stream */

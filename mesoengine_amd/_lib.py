@@ -137,6 +137,7 @@ _SIGS = {
     "meso_event_create": (C.c_int32, [P, PP]),
     "meso_event_destroy": (C.c_int32, [P]),
     "meso_event_record": (C.c_int32, [P, P]),
+    "meso_stream_wait_event": (C.c_int32, [P, P, P]),
     "meso_event_sync": (C.c_int32, [P]),
     "meso_event_elapsed_ms": (C.c_int32, [P, P, C.POINTER(C.c_float)]),
     "meso_memcpy_d2h": (C.c_int32, [P, P, P, C.c_uint64, P]),

@@ -596,6 +596,11 @@ meso_status meso_event_record(meso_event *ev, meso_stream stream) {
   MESO_HIP(hipEventRecord(ev->ev, stream_of(ev->ctx, stream)));
   return MESO_OK;
 }
+meso_status meso_stream_wait_event(meso_context *ctx, meso_stream stream, meso_event *ev) {
+  MESO_CHECK(ctx && ev, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
+  MESO_HIP(hipStreamWaitEvent(stream_of(ctx, stream), ev->ev, 0));
+  return MESO_OK;
+}
 meso_status meso_event_sync(meso_event *ev) {
   MESO_CHECK(ev, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL event");
   MESO_HIP(hipEventSynchronize(ev->ev));

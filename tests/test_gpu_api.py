@@ -240,6 +240,36 @@ def test_host_bricks_unreferenced_and_held_twice(ctx):
     assert b"two cells" in ctx.lib.meso_last_error_message()
 
 
+def test_frames_in_flight_on_two_streams(ctx):
+    """Consecutive frames on two streams, ordered by events, each into its own
+    outputs, equal the frame rendered alone."""
+    sc = ctx.scene_generate(L.GEN_HEIGHTFIELD, L.SCENE_BRICKMAP_U16, 256, 2)
+    cp = W.terrain_camera(256, 320, 200)
+    cam = R.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], 320, 200)
+    ref = R.RayTargets(ctx, 320, 200)
+    R.raymarch(ctx, sc, cam, L.RAY_SHADOW, R.shading(W.LIGHT), ref)
+    ctx.sync()
+    want = ref.results()
+    s2 = C.c_void_p()
+    L.check(ctx.lib.meso_stream_create(ctx.h, C.byref(s2)))
+    tgs = [R.RayTargets(ctx, 320, 200), R.RayTargets(ctx, 320, 200)]
+    streams = [None, s2.value]
+    start, mid = R.Event(ctx), R.Event(ctx)
+    start.record()
+    L.check(ctx.lib.meso_stream_wait_event(ctx.h, s2, start.h))
+    for i in range(6):
+        R.raymarch(ctx, sc, cam, L.RAY_SHADOW, R.shading(W.LIGHT), tgs[i % 2], streams[i % 2])
+    mid.record(s2)
+    L.check(ctx.lib.meso_stream_wait_event(ctx.h, None, mid.h))
+    ctx.sync()
+    for tg in tgs:
+        for x, y in zip(tg.results(), want):
+            assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
+    assert ctx.lib.meso_stream_wait_event(ctx.h, s2, None) == L.ERR_ARG
+    L.check(ctx.lib.meso_stream_destroy(ctx.h, s2))
+    sc.destroy()
+
+
 def test_argument_errors(ctx):
     lib = ctx.lib
     sc = ctx.scene_generate(L.GEN_SPHERES, L.SCENE_DENSE_U8, 64, 3, 4)
