@@ -64,6 +64,52 @@ def raymarch_leg(ctx, R, L, W, steps, warmup):
                                            L.byref(tg.out), L.byref(tune)))
     log("raymarch tune: variant %d (grid %d, tile %dx%d) %.3f ms, default %.3f ms" % (
         tune.variant, tune.grid, tune.tile_w, tune.tile_h, tune.best_ms, tune.default_ms))
+    # the headline runs two frames in flight (below), where the shapes rank
+    # differently (tiles overlap worse than persistent packets), so the
+    # candidate shapes are timed that way too and the fastest is pinned for
+    # every timed frame of this leg
+    s2 = C.c_void_p()
+    L.check(ctx.lib.meso_stream_create(ctx.h, C.byref(s2)))
+    tg2 = R.RayTargets(ctx, w, h)
+    streams, targets = [None, s2.value], [tg, tg2]
+
+    def two_in_flight(k):
+        ctx.sync()
+        L.check(ctx.lib.meso_stream_sync(ctx.h, s2))
+        a, b, mid = R.Event(ctx), R.Event(ctx), R.Event(ctx)
+        a.record()
+        L.check(ctx.lib.meso_stream_wait_event(ctx.h, s2, a.h))
+        for i in range(k):
+            R.raymarch(ctx, sc, cam, flags, sh, targets[i % 2], streams[i % 2])
+        mid.record(s2)
+        L.check(ctx.lib.meso_stream_wait_event(ctx.h, None, mid.h))
+        b.record()
+        b.sync()
+        L.check(ctx.lib.meso_stream_sync(ctx.h, s2))
+        return a.elapsed_ms(b) / k
+
+    tiles = [(16, 16), (32, 8), (8, 8), (16, 8), (32, 16), (8, 8), (16, 4), (32, 2)]
+    cus = ctx.device_info().compute_units
+    shapes = [(5, 0), (5, 12 * cus), (5, 16 * cus), (5, 20 * cus), (5, 24 * cus), (6, 16 * cus),
+              (6, 24 * cus), (2, 0), (0, 0), (tune.variant, tune.grid)]
+    best = None
+    for v, g in shapes:
+        pin = L.TuneResult()
+        pin.size = C.sizeof(pin)
+        pin.variant, pin.grid = v, g
+        pin.tile_w, pin.tile_h = tiles[v]
+        L.check(ctx.lib.meso_autotune_pin_raymarch(ctx.h, sc.h, L.byref(cam), flags, L.byref(pin)))
+        two_in_flight(4)
+        t_pipe = min(two_in_flight(12), two_in_flight(12))
+        if best is None or t_pipe < best[0]:
+            best = (t_pipe, v, g)
+    pin = L.TuneResult()
+    pin.size = C.sizeof(pin)
+    pin.variant, pin.grid = best[1], best[2]
+    pin.tile_w, pin.tile_h = tiles[best[1]]
+    L.check(ctx.lib.meso_autotune_pin_raymarch(ctx.h, sc.h, L.byref(cam), flags, L.byref(pin)))
+    log("raymarch tune, two frames in flight: variant %d grid %d, %.3f ms" % (
+        best[1], best[2], best[0]))
     # untimed counting pass: the algorithmic byte model needs the step counts
     R.raymarch(ctx, sc, cam, flags | L.RAY_COUNT, sh, tg)
     ctx.sync()
@@ -84,10 +130,6 @@ def raymarch_leg(ctx, R, L, W, steps, warmup):
     # outputs, so one frame's last packets overlap the next frame's first.
     # The second stream waits for the start event and the end event waits for
     # the second stream, so the interval holds every frame's whole work.
-    s2 = C.c_void_p()
-    L.check(ctx.lib.meso_stream_create(ctx.h, C.byref(s2)))
-    tg2 = R.RayTargets(ctx, w, h)
-    streams, targets = [None, s2.value], [tg, tg2]
     for i in range(2 * warmup):
         R.raymarch(ctx, sc, cam, flags, sh, targets[i % 2], streams[i % 2])
     ctx.sync()
@@ -132,7 +174,9 @@ def raymarch_leg(ctx, R, L, W, steps, warmup):
                layout_bytes=layout_bytes,
                mrays=rays / (ms * 1e-3) / 1e6, mrays_primary=cnt["rays"] / (ms * 1e-3) / 1e6,
                tune=dict(variant=tune.variant, grid=tune.grid, tile=[tune.tile_w, tune.tile_h],
-                         best_ms=round(tune.best_ms, 4), default_ms=round(tune.default_ms, 4)),
+                         best_ms=round(tune.best_ms, 4), default_ms=round(tune.default_ms, 4),
+                         two_in_flight=dict(variant=best[1], grid=best[2],
+                                            tile=list(tiles[best[1]]), ms=round(best[0], 4))),
                pcie_inclusive=pcie)
     sc.destroy()
     return res
@@ -431,8 +475,9 @@ def main():
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tp):
-        # the PMC traffic of the launch shape the autotuner picked here
-        shape = {5: "p8x8", 6: "p16x4", 7: "p32x2"}.get(rm["tune"]["variant"], "tiled")
+        # the PMC traffic of the launch shape pinned for the timed frames
+        shape = {5: "p8x8", 6: "p16x4", 7: "p32x2"}.get(rm["tune"]["two_in_flight"]["variant"],
+                                                         "tiled")
         with open(tp) as f:
             k = json.load(f).get("kernels", {}).get("raymarch_brickmap_terrain512_" + shape)
             traffic = k["hbm_bytes"] if k else None
