@@ -88,10 +88,11 @@ def raymarch_leg(ctx, R, L, W, steps, warmup):
         L.check(ctx.lib.meso_stream_sync(ctx.h, s2))
         return a.elapsed_ms(b) / k
 
-    tiles = [(16, 16), (32, 8), (8, 8), (16, 8), (32, 16), (8, 8), (16, 4), (32, 2)]
+    tiles = [(16, 16), (32, 8), (8, 8), (16, 8), (32, 16), (8, 8), (16, 4), (32, 2), (8, 8),
+             (16, 4)]
     cus = ctx.device_info().compute_units
     shapes = [(5, 0), (5, 12 * cus), (5, 16 * cus), (5, 20 * cus), (5, 24 * cus), (6, 16 * cus),
-              (6, 24 * cus), (2, 0), (0, 0), (tune.variant, tune.grid)]
+              (6, 24 * cus), (2, 0), (0, 0), (8, 0), (8, 16 * cus), (tune.variant, tune.grid)]
     best = None
     for v, g in shapes:
         pin = L.TuneResult()
@@ -476,8 +477,8 @@ def main():
     tp = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tp):
         # the PMC traffic of the launch shape pinned for the timed frames
-        shape = {5: "p8x8", 6: "p16x4", 7: "p32x2"}.get(rm["tune"]["two_in_flight"]["variant"],
-                                                         "tiled")
+        shape = {5: "p8x8", 6: "p16x4", 7: "p32x2", 8: "pair8x8", 9: "pair16x4"}.get(
+            rm["tune"]["two_in_flight"]["variant"], "tiled")
         with open(tp) as f:
             k = json.load(f).get("kernels", {}).get("raymarch_brickmap_terrain512_" + shape)
             traffic = k["hbm_bytes"] if k else None

@@ -377,10 +377,12 @@ meso_status meso_autotune_raymarch(meso_context *ctx, meso_scene *scene, const m
                                    const meso_ray_outputs *out, meso_tune_result *result);
 /* Pins a raymarch launch shape for the workload key (scene kind and edge,
  * frame size, flags without MESO_RAY_COUNT), replacing the tuner's choice:
- * shape->variant 0..4 (tiles 16x16, 32x8, 8x8, 16x8, 32x16) or 5..7
- * (persistent waves on 8x8, 16x4, 32x2 packets; shape->grid workgroups of
- * 64 lanes, 0 = one per wave slot, at most 32 per CU); tile_w / tile_h must
- * match the variant's tile or packet.  A caller that shares the
+ * shape->variant 0..4 (tiles 16x16, 32x8, 8x8, 16x8, 32x16), 5..7
+ * (persistent waves on 8x8, 16x4, 32x2 packets) or 8..9 (persistent waves
+ * over pairs of 8x8 or 16x4 packets, their secondary rays shared over the
+ * wave); for 5..9 shape->grid is the number of workgroups of 64 lanes, 0 =
+ * one per wave slot, at most 32 per CU; tile_w / tile_h must match the
+ * variant's tile or packet.  A caller that shares the
  * device with latency-bound work pins a smaller persistent grid so that
  * the other work keeps wave slots.  No lvk counterpart. */
 meso_status meso_autotune_pin_raymarch(meso_context *ctx, meso_scene *scene, const meso_camera *cam,

@@ -563,16 +563,14 @@ __device__ __forceinline__ float shade_value(float lambert, float vis, bool ao_o
   return shade;
 }
 
-// Secondary rays of a primary hit (voxel `cell`, entry face `face` < 6, at t
-// along o + t d): the shadow ray toward the light when the face is lit, the
-// AO rays, and the shade.  With neither flag set it traces nothing.
+// The secondary rays of pixel `pix` whose primary ray hit face `face` (< 6),
+// set up by secondary_setup: the shadow ray toward the light when the face is
+// lit, the AO rays, and the shade.  With neither flag set it traces nothing.
 template <int KIND, bool COUNT>
-__device__ __forceinline__ float shade_hit(const DevScene &s, uint32_t flags, const DevShading &sh,
-                                           uint32_t pix, const float o[3], const float d[3],
-                                           int32_t face, float t, const int32_t cell[3],
-                                           StepCounts &cnt, uint32_t &n_shadow, uint32_t &n_ao) {
-  Secondary sc;
-  secondary_setup(sh, o, d, face, t, cell, sc);
+__device__ __forceinline__ float shade_secondary(const DevScene &s, uint32_t flags,
+                                                 const DevShading &sh, uint32_t pix, int32_t face,
+                                                 const Secondary &sc, StepCounts &cnt,
+                                                 uint32_t &n_shadow, uint32_t &n_ao) {
   float vis = 1.0f;
   if ((flags & MESO_RAY_SHADOW) && sc.lambert > 0.0f) {
     n_shadow++;
@@ -588,6 +586,18 @@ __device__ __forceinline__ float shade_hit(const DevScene &s, uint32_t flags, co
     }
   }
   return shade_value(sc.lambert, vis, (flags & MESO_RAY_AO) != 0, unocc, sh.ao_samples);
+}
+
+// Secondary rays of a primary hit (voxel `cell`, entry face `face` < 6, at t
+// along o + t d) and the shade.
+template <int KIND, bool COUNT>
+__device__ __forceinline__ float shade_hit(const DevScene &s, uint32_t flags, const DevShading &sh,
+                                           uint32_t pix, const float o[3], const float d[3],
+                                           int32_t face, float t, const int32_t cell[3],
+                                           StepCounts &cnt, uint32_t &n_shadow, uint32_t &n_ao) {
+  Secondary sc;
+  secondary_setup(sh, o, d, face, t, cell, sc);
+  return shade_secondary<KIND, COUNT>(s, flags, sh, pix, face, sc, cnt, n_shadow, n_ao);
 }
 
 // true when a primary hit on `face` needs secondary rays

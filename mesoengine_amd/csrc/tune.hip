@@ -193,14 +193,16 @@ meso_status meso_autotune_raymarch(meso_context *ctx, meso_scene *scene, const m
     derive_shading(shading, &sh);
   RayOut ro{outp->hit_index, outp->index_bytes, outp->face, outp->t, outp->shade, nullptr};
   // tiled shapes, then persistent waves with 8x8, 16x4 and 32x2 packets at
-  // full occupancy (grid 0) and at 24 (16) waves per CU
+  // full occupancy (grid 0) and at 24 (16) waves per CU, then waves over
+  // pairs of 8x8 packets
   const int cus = ctx->props.multiProcessorCount;
   struct Shape {
     int v, tw, th, grid;
   } shapes[] = {{0, 16, 16, 0},      {1, 32, 8, 0},       {2, 8, 8, 0},
                 {3, 16, 8, 0},       {4, 32, 16, 0},      {5, 8, 8, 0},
                 {5, 8, 8, 24 * cus}, {5, 8, 8, 16 * cus}, {6, 16, 4, 0},
-                {6, 16, 4, 24 * cus}, {7, 32, 2, 0},      {7, 32, 2, 24 * cus}};
+                {6, 16, 4, 24 * cus}, {7, 32, 2, 0},      {7, 32, 2, 24 * cus},
+                {8, 8, 8, 0},        {8, 8, 8, 16 * cus}};
   meso_tune_result best{};
   best.size = sizeof(best);
   best.best_ms = 1e30f;
@@ -239,10 +241,10 @@ meso_status meso_autotune_pin_raymarch(meso_context *ctx, meso_scene *scene, con
   MESO_CHECK(ctx && scene && cam && shape, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL argument");
   MESO_CHECK(cam->size >= sizeof(meso_camera) && shape->size >= sizeof(meso_tune_result),
              MESO_ERR_ARGUMENT_OUT_OF_RANGE, "struct size too small");
-  static const int tiles[8][2] = {{16, 16}, {32, 8}, {8, 8}, {16, 8}, {32, 16},
-                                  {8, 8},   {16, 4}, {32, 2}};
+  static const int tiles[10][2] = {{16, 16}, {32, 8}, {8, 8}, {16, 8}, {32, 16},
+                                   {8, 8},   {16, 4}, {32, 2}, {8, 8},  {16, 4}};
   meso_tune_result r = *shape;
-  MESO_CHECK(r.variant >= 0 && r.variant <= 7, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+  MESO_CHECK(r.variant >= 0 && r.variant <= 9, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
              "unknown raymarch variant %d", r.variant);
   MESO_CHECK(r.tile_w == tiles[r.variant][0] && r.tile_h == tiles[r.variant][1],
              MESO_ERR_ARGUMENT_OUT_OF_RANGE, "variant %d has %dx%d %s", r.variant,
@@ -293,7 +295,7 @@ meso_status meso_autotune_load(meso_context *ctx, const char *path) {
     if (!(is >> key >> r.variant >> r.block >> r.grid >> r.tile_w >> r.tile_h >> r.best_ms >>
           r.default_ms))
       continue;
-    if (r.variant < 0 || r.variant > 7 || r.grid < 0 ||
+    if (r.variant < 0 || r.variant > 9 || r.grid < 0 ||
         r.grid > 32 * ctx->props.multiProcessorCount)
       continue;
     ctx->tune[key] = r;

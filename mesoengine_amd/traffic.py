@@ -39,10 +39,14 @@ KERNELS = {
     "void meso::raymarch_persistent_kernel<2, false, 16>": "raymarch_brickmap_terrain512_p16x4",
     "void meso::raymarch_persistent_kernel<2, false, 32>": "raymarch_brickmap_terrain512_p32x2",
     "void meso::raymarch_kernel<2, false,": "raymarch_brickmap_terrain512_tiled",
+    "void meso::raymarch_pair_kernel<2, false, 8>": "raymarch_brickmap_terrain512_pair8x8",
+    "void meso::raymarch_pair_kernel<2, false, 16>": "raymarch_brickmap_terrain512_pair16x4",
     "void meso::raymarch_persistent_kernel<3, false, 8>": "raymarch_two_level_caves2048_p8x8",
     "void meso::raymarch_persistent_kernel<3, false, 16>": "raymarch_two_level_caves2048_p16x4",
     "void meso::raymarch_persistent_kernel<3, false, 32>": "raymarch_two_level_caves2048_p32x2",
     "void meso::raymarch_kernel<3, false,": "raymarch_two_level_caves2048_tiled",
+    "void meso::raymarch_pair_kernel<3, false, 8>": "raymarch_two_level_caves2048_pair8x8",
+    "void meso::raymarch_pair_kernel<3, false, 16>": "raymarch_two_level_caves2048_pair16x4",
     "void meso::mesh_batched_kernel<2, true, 0>": "mesh_batched_greedy_4096",
     "void meso::mesh_batched_kernel<2, false, 0>": "mesh_batched_culled_4096",
 }

@@ -117,7 +117,7 @@ def test_autotuners_keep_results(ctx, tmp_path):
     rr.size = C.sizeof(rr)
     L.check(ctx.lib.meso_autotune_raymarch(ctx.h, sc.h, L.byref(cam), L.RAY_SHADOW,
                                            L.byref(R.shading()), L.byref(tg.out), L.byref(rr)))
-    assert rr.tried == 12 and rr.tile_w * rr.tile_h == rr.block
+    assert rr.tried == 14 and rr.tile_w * rr.tile_h == rr.block
     R.raymarch(ctx, sc, cam, L.RAY_SHADOW, R.shading(), tg)
     ctx.sync()
     for a, b in zip(ref, tg.results()):

@@ -1,8 +1,9 @@
 """Every raymarch launch shape gives the oracle's bits and step counts.
 
-The five tiled shapes and the persistent waves on 8x8, 16x4 and 32x2
-packets (variants 5-7, at the device's full occupancy and at a small grid
-that makes every wave serve several bands) are forced through the
+The five tiled shapes, the persistent waves on 8x8, 16x4 and 32x2 packets
+(variants 5-7) and the waves over pairs of packets whose secondary rays are
+shared over the wave (variants 8-9), at the device's full occupancy and at a
+small grid that makes every wave serve several bands, are forced through the
 autotuner's cache file, then run through meso_raymarch with the step
 counters on.
 """
@@ -18,7 +19,8 @@ pytestmark = pytest.mark.gpu
 
 # (variant, tile_w, tile_h, grid)
 SHAPES = [(0, 16, 16, 0), (1, 32, 8, 0), (2, 8, 8, 0), (3, 16, 8, 0), (4, 32, 16, 0),
-          (5, 8, 8, 0), (5, 8, 8, 64), (6, 16, 4, 0), (7, 32, 2, 64)]
+          (5, 8, 8, 0), (5, 8, 8, 64), (6, 16, 4, 0), (7, 32, 2, 64), (8, 8, 8, 0),
+          (8, 8, 8, 64), (9, 16, 4, 64)]
 
 
 def _force(ctx, tmp_path, kind, n, w, h, flags, shape):
@@ -59,7 +61,8 @@ def test_brickmap_shapes_match_oracle(ctx, tmp_path, shape):
     sc.destroy()
 
 
-@pytest.mark.parametrize("shape", [(5, 8, 8, 0), (5, 8, 8, 24), (6, 16, 4, 24), (7, 32, 2, 0)])
+@pytest.mark.parametrize("shape", [(5, 8, 8, 0), (5, 8, 8, 24), (6, 16, 4, 24), (7, 32, 2, 0),
+                                   (8, 8, 8, 0), (8, 8, 8, 24), (9, 16, 4, 0)])
 def test_persistent_two_level_ao_and_dense(ctx, tmp_path, shape):
     n, w, h = 256, 136, 72
     sc = ctx.scene_generate(L.GEN_CAVES, L.SCENE_TWO_LEVEL_U16, n, 5)
@@ -104,7 +107,7 @@ def test_pin_validates_shapes(ctx):
     assert ctx.lib.meso_autotune_pin_raymarch(ctx.h, sc.h, L.byref(cam), 0, L.byref(t)) == L.ERR_ARG
     t.variant, t.grid = 5, 33 * ctx.device_info().compute_units
     assert ctx.lib.meso_autotune_pin_raymarch(ctx.h, sc.h, L.byref(cam), 0, L.byref(t)) == L.ERR_ARG
-    t.variant = 9
+    t.variant = 10
     assert ctx.lib.meso_autotune_pin_raymarch(ctx.h, sc.h, L.byref(cam), 0, L.byref(t)) == L.ERR_ARG
     # a valid pin renders the same bits as the default shape
     tg0 = R.RayTargets(ctx, 64, 48)
