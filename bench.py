@@ -92,7 +92,8 @@ def raymarch_leg(ctx, R, L, W, steps, warmup):
              (16, 4)]
     cus = ctx.device_info().compute_units
     shapes = [(5, 0), (5, 12 * cus), (5, 16 * cus), (5, 20 * cus), (5, 24 * cus), (6, 16 * cus),
-              (6, 24 * cus), (2, 0), (0, 0), (8, 0), (8, 16 * cus), (tune.variant, tune.grid)]
+              (6, 24 * cus), (2, 0), (0, 0), (8, 0), (8, 16 * cus), (8, 20 * cus),
+              (tune.variant, tune.grid)]
     best = None
     for v, g in shapes:
         pin = L.TuneResult()
