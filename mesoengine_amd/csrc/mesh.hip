@@ -455,49 +455,52 @@ __device__ __forceinline__ int plane_of(const MeshLds &L, uint32_t q) {
   return lo;
 }
 
-// Culled emission balanced over the whole workgroup: thread i writes the
-// quads [total*i/n, total*(i+1)/n) of the chunk, in canonical order (plane,
-// row, u), so every thread loops the same number of times whatever the
-// spread of faces over the planes.  A thread finds its first face by a binary
-// search over the planes' offsets (pex) and a walk over the plane's non-empty
-// rows (occ); it crosses to the next plane by the same search.
+// Culled emission spread over the whole workgroup in runs of kRun quads:
+// thread i writes the runs [kRun (i + n k), kRun (i + n k + 1)) of the chunk
+// (n = workgroup size, k = 0, 1, ...), each in canonical order (plane, row,
+// u), so one store instruction covers n * kRun consecutive quads (coalesced)
+// and every thread loops about the same number of times whatever the spread
+// of faces over the planes.  A thread finds the first face of a run by a
+// binary search over the planes' offsets (pex) and a walk over the plane's
+// non-empty rows (occ); it crosses to the next plane by the same search.
+constexpr uint32_t kRun = 8;
+
 __device__ __forceinline__ void culled_emit_balanced(const MeshLds &L, uint32_t total,
                                                      uint32_t *out, uint64_t base, uint64_t cap) {
-  const uint32_t t = threadIdx.x;
-  const uint32_t q0 = (uint32_t)(((uint64_t)total * t) / kMeshBlock);
-  const uint32_t q1 = (uint32_t)(((uint64_t)total * (t + 1u)) / kMeshBlock);
-  if (q0 >= q1) return;
-  int pl = plane_of(L, q0);
-  PlaneView p = plane_view(pl >> 5, pl & 31);
-  uint32_t r = q0 - L.pex[pl];
-  uint32_t rest = L.occ[pl];  // non-empty rows not yet entered
-  int v = __ffs(rest) - 1;
-  rest &= rest - 1u;
-  uint32_t row = face_row(L, p, v);
-  for (;;) {  // the row holding face r of the plane
-    const uint32_t c = (uint32_t)__popc(row);
-    if (r < c) break;
-    r -= c;
-    v = __ffs(rest) - 1;
+  for (uint32_t q0 = threadIdx.x * kRun; q0 < total; q0 += kMeshBlock * kRun) {
+    const uint32_t q1 = q0 + kRun < total ? q0 + kRun : total;
+    int pl = plane_of(L, q0);
+    PlaneView p = plane_view(pl >> 5, pl & 31);
+    uint32_t r = q0 - L.pex[pl];
+    uint32_t rest = L.occ[pl];  // non-empty rows not yet entered
+    int v = __ffs(rest) - 1;
     rest &= rest - 1u;
-    row = face_row(L, p, v);
-  }
-  row &= ~((1u << select_bit(row, r)) - 1u);  // drop the faces before q0
-  for (uint32_t q = q0;;) {
-    const int u = __ffs(row) - 1;
-    row &= row - 1u;
-    const uint64_t k = base + q;
-    if (k < cap) out[k] = pack_quad(pl >> 5, pl & 31, u, v, 1, 1);
-    if (++q >= q1) break;
-    if (row == 0u) {  // next non-empty row, else the plane holding quad q
-      if (rest == 0u) {
-        pl = plane_of(L, q);
-        p = plane_view(pl >> 5, pl & 31);
-        rest = L.occ[pl];
-      }
+    uint32_t row = face_row(L, p, v);
+    for (;;) {  // the row holding face r of the plane
+      const uint32_t c = (uint32_t)__popc(row);
+      if (r < c) break;
+      r -= c;
       v = __ffs(rest) - 1;
       rest &= rest - 1u;
       row = face_row(L, p, v);
+    }
+    row &= ~((1u << select_bit(row, r)) - 1u);  // drop the faces before q0
+    for (uint32_t q = q0;;) {
+      const int u = __ffs(row) - 1;
+      row &= row - 1u;
+      const uint64_t k = base + q;
+      if (k < cap) out[k] = pack_quad(pl >> 5, pl & 31, u, v, 1, 1);
+      if (++q >= q1) break;
+      if (row == 0u) {  // next non-empty row, else the plane holding quad q
+        if (rest == 0u) {
+          pl = plane_of(L, q);
+          p = plane_view(pl >> 5, pl & 31);
+          rest = L.occ[pl];
+        }
+        v = __ffs(rest) - 1;
+        rest &= rest - 1u;
+        row = face_row(L, p, v);
+      }
     }
   }
 }
