@@ -283,22 +283,25 @@ __device__ __forceinline__ uint32_t pack_quad(int f, int s, int u, int v, int w,
          ((uint32_t)(h - 1) << 20) | ((uint32_t)f << 25);
 }
 
+// x and every bit reachable from it by moving up through bits of g (bit j+1
+// reachable from bit j when bit j+1 is in g).  A seed just above each bit of
+// x that lies in g is added to g: the addition turns the run of g's ones from
+// the seed to the run's top into zeros and leaves the carry on the first bit
+// above the run, which is outside g; so (g + s) ^ g marks the run from the
+// seed up (plus that bit, masked off by & g).  A second seed in the same run
+// is already inside the flipped stretch except for its own bit, added by | s.
+__device__ __forceinline__ uint32_t fill_up(uint32_t x, uint32_t g) {
+  const uint32_t s = (x << 1) & g;
+  return x | ((((g + s) ^ g) | s) & g);
+}
+
 // Bits of the segments that contain a marked bit.  Segments are runs of
 // `cont` bits (a bit continuing the segment of the bit below it) above a start
-// bit; the marks spread up and down through them (prefix-OR by doubling).
+// bit; the marks spread up through `cont`, and down (bit j reached from bit
+// j+1 when bit j+1 is in `cont`) by the same fill on the bit-reversed words.
 __device__ __forceinline__ uint32_t segment_fill(uint32_t marks, uint32_t cont) {
-  uint32_t up = marks, g = cont;
-#pragma unroll
-  for (int k = 1; k < 32; k <<= 1) {
-    up |= (up << k) & g;
-    g &= g << k;
-  }
-  uint32_t dn = marks, gd = cont >> 1;
-#pragma unroll
-  for (int k = 1; k < 32; k <<= 1) {
-    dn |= (dn >> k) & gd;
-    gd &= gd >> k;
-  }
+  const uint32_t up = fill_up(marks, cont);
+  const uint32_t dn = __brev(fill_up(__brev(marks), __brev(cont) << 1));
   return up | dn;
 }
 
