@@ -118,23 +118,27 @@ __global__ void pool_begin_kernel(uint32_t *counters, uint32_t limit) {
   counters[kCntStamp] = counters[kCntStamp] + 1u;
 }
 
-// Candidate keys, one per offset of the (2R+1)^3 cube: importance descending
-// (complemented float bits; importances are positive), then chunk index;
-// offsets that are no candidate get the largest key and sort last.
+// Candidate keys: importance descending (complemented float bits;
+// importances are positive), then chunk index.  With `compact` the
+// candidates are appended to `keys` in any order (their number is the
+// candidate counter); otherwise there is one key per offset of the
+// (2R+1)^3 cube and offsets that are no candidate get the largest key, so
+// that a full sort puts them last.
 template <int KIND>
 __global__ void __launch_bounds__(256) pool_rank_kernel(DevScene s, PoolGeom g, int32_t cx0,
                                                         int32_t cy0, int32_t cz0, float f0,
                                                         float f1, float f2,
                                                         const uint32_t *lookup, uint64_t *keys,
-                                                        uint64_t cube, uint32_t *counters) {
+                                                        uint64_t cube, uint32_t *counters,
+                                                        bool compact) {
   const uint64_t k = block_id() * 256u + threadIdx.x;
   bool cand = false;
+  uint64_t key = ~0ull;
   if (k < cube) {
     const uint32_t side = (uint32_t)g.side;
     const int32_t dx = (int32_t)(k % side) - g.R, dy = (int32_t)((k / side) % side) - g.R,
                   dz = (int32_t)(k / ((uint64_t)side * side)) - g.R;
     const int32_t cx = cx0 + dx, cy = cy0 + dy, cz = cz0 + dz;
-    uint64_t key = ~0ull;
     if (cx >= 0 && cy >= 0 && cz >= 0 && cx < g.nc && cy < g.nc && cz < g.nc) {
       const uint32_t idx = (uint32_t)cx + (uint32_t)g.nc * ((uint32_t)cy + (uint32_t)g.nc * (uint32_t)cz);
       if (lookup[idx] == kNoSlot && chunk_desired(dx, dy, dz, f0, f1, f2, g) &&
@@ -144,10 +148,16 @@ __global__ void __launch_bounds__(256) pool_rank_kernel(DevScene s, PoolGeom g, 
         cand = true;
       }
     }
-    keys[k] = key;
+    if (!compact) keys[k] = key;
   }
   const uint64_t b = __ballot(cand);
-  if ((threadIdx.x & 63u) == 0 && b) atomicAdd(&counters[kCntCandidates], (uint32_t)__popcll(b));
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t base = 0u;
+  if (lane == 0u && b) base = atomicAdd(&counters[kCntCandidates], (uint32_t)__popcll(b));
+  if (compact) {
+    base = __shfl(base, 0);
+    if (cand) keys[base + (uint32_t)__popcll(b & ((1ull << lane) - 1ull))] = key;
+  }
 }
 
 // Victim keys, one per slot: free slots first (by slot), then resident
@@ -167,6 +177,155 @@ __global__ void __launch_bounds__(256) pool_victim_kernel(PoolGeom g, int32_t cx
   const int32_t x = (int32_t)(c % nc), y = (int32_t)((c / nc) % nc), z = (int32_t)(c / (nc * nc));
   const float imp = chunk_importance(x - cx0, y - cy0, z - cz0, f0, f1, f2);
   keys[sl] = ((uint64_t)__float_as_uint(imp) << 32) | sl;
+}
+
+// The first `limit` keys of a list in ascending order, for limits up to
+// kSelMax: one workgroup per list, block 0 the compact candidates (their
+// number is the candidate counter), block 1 the victims (one per slot).
+// An MSB-first radix select over 8-bit digits finds t, the limit-th
+// smallest key: per digit a histogram in LDS of the keys that share the
+// digits found so far, and a scan by one wave.  The keys below t are
+// gathered into LDS, the remaining places up to the limit take t, and a
+// bitonic sort orders the kpad (power of two >= limit) places.  That is
+// the first `limit` places of a full sort of the list, repeated keys
+// included.  A list no longer than the limit is taken whole and padded
+// with the largest key, which is what a full sort of the cube's keys puts
+// after the candidates.  The first kSelPer * kSelThreads keys stay in
+// registers across the passes; the rest are re-read from memory.
+constexpr uint32_t kSelThreads = 1024u, kSelMax = 4096u, kSelPer = 16u;
+
+__global__ void __launch_bounds__(kSelThreads)
+    pool_select_kernel(const uint64_t *cand, const uint64_t *vict, uint32_t cap, uint32_t limit,
+                       uint32_t kpad, const uint32_t *counters, uint64_t *cand_out,
+                       uint64_t *vict_out) {
+  __shared__ uint64_t buf[kSelMax];
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t s_bin, s_below, s_cnt, s_fill;
+  __shared__ uint64_t s_key;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
+  const bool victims = blockIdx.x != 0u;
+  const uint64_t *keys = victims ? vict : cand;
+  uint64_t *out = victims ? vict_out : cand_out;
+  const uint32_t n = victims ? cap : counters[kCntCandidates];
+  constexpr uint32_t kHeld = kSelPer * kSelThreads;
+  uint64_t r[kSelPer];
+#pragma unroll
+  for (uint32_t j = 0; j < kSelPer; j++) {
+    const uint32_t i = tid + j * kSelThreads;
+    r[j] = i < n ? keys[i] : 0ull;
+  }
+  // f(key, valid) for every key of the list; each call is made by all lanes
+  // of the workgroup together, so f may use wave-wide operations
+  auto each = [&](auto &&f) {
+#pragma unroll
+    for (uint32_t j = 0; j < kSelPer; j++) f(r[j], tid + j * kSelThreads < n);
+    for (uint32_t b = kHeld; b < n; b += kSelThreads) {
+      const uint32_t i = b + tid;
+      f(i < n ? keys[i] : 0ull, i < n);
+    }
+  };
+  uint64_t t = ~0ull;     // the limit-th smallest key
+  uint32_t rem = limit;   // its rank among the keys sharing the digits found
+  if (n > limit) {
+    uint64_t prefix = 0ull, hi = 0ull;  // digits found, and their bits
+    int shift = 56;
+    bool alone = false;
+    for (;; shift -= 8) {
+      if (tid < 256u) hist[tid] = 0u;
+      __syncthreads();
+      each([&](uint64_t key, bool valid) {
+        const bool m = valid && (key & hi) == prefix;
+        const uint32_t d = (uint32_t)(key >> shift) & 255u;
+        const uint64_t mb = __ballot(m);
+        if (!mb) return;
+        // one atomic for a wave whose matching keys share the digit
+        const uint32_t first = (uint32_t)__ffsll((long long)mb) - 1u;
+        const uint32_t d0 = __shfl(d, (int)first);
+        if (__ballot(m && d == d0) == mb) {
+          if (lane == first) atomicAdd(&hist[d0], (uint32_t)__popcll(mb));
+        } else if (m) {
+          atomicAdd(&hist[d], 1u);
+        }
+      });
+      __syncthreads();
+      if (tid < 64u) {
+        const uint32_t h0 = hist[4u * tid], h1 = hist[4u * tid + 1u], h2 = hist[4u * tid + 2u],
+                       h3 = hist[4u * tid + 3u];
+        const uint32_t c = h0 + h1 + h2 + h3;
+        uint32_t inc = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t v = __shfl_up(inc, o);
+          if ((int)tid >= o) inc += v;
+        }
+        // the bins hold at least rem keys in all, so some lane reaches rem
+        const uint64_t over = __ballot(inc >= rem);
+        const uint32_t at = over ? (uint32_t)__ffsll((long long)over) - 1u : 63u;
+        if (tid == at) {
+          uint32_t cum = inc - c, bin = 4u * tid, hb = h0;
+          if (cum + h0 < rem) {
+            cum += h0, bin++, hb = h1;
+            if (cum + h1 < rem) {
+              cum += h1, bin++, hb = h2;
+              if (cum + h2 < rem) cum += h2, bin++, hb = h3;
+            }
+          }
+          s_bin = bin;
+          s_below = cum;
+          s_cnt = hb;
+        }
+      }
+      __syncthreads();
+      prefix |= (uint64_t)s_bin << shift;
+      hi |= 255ull << shift;
+      rem -= s_below;
+      if (shift == 0) break;
+      // one key holds the digits found: it is t
+      if (s_cnt == 1u) {
+        alone = true;
+        break;
+      }
+    }
+    if (alone) {
+      each([&](uint64_t key, bool valid) {
+        if (valid && (key & hi) == prefix) s_key = key;
+      });
+      __syncthreads();
+      t = s_key;
+    } else {
+      t = prefix;
+    }
+  }
+  // keys below t: n when the list is taken whole, else limit - rem
+  const uint32_t below = n > limit ? limit - rem : n;
+  if (tid == 0u) s_fill = 0u;
+  for (uint32_t i = below + tid; i < kpad; i += kSelThreads) buf[i] = i < limit ? t : ~0ull;
+  __syncthreads();
+  each([&](uint64_t key, bool valid) {
+    const bool m = valid && key < t;
+    const uint64_t mb = __ballot(m);
+    if (!mb) return;
+    uint32_t base = 0u;
+    if (lane == 0u) base = atomicAdd(&s_fill, (uint32_t)__popcll(mb));
+    base = __shfl(base, 0);
+    const uint32_t pos = base + (uint32_t)__popcll(mb & ((1ull << lane) - 1ull));
+    if (m && pos < below) buf[pos] = key;
+  });
+  for (uint32_t size = 2u; size <= kpad; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0u; stride >>= 1) {
+      __syncthreads();
+      for (uint32_t q = tid; q < kpad / 2u; q += kSelThreads) {
+        const uint32_t i = 2u * q - (q & (stride - 1u)), j = i + stride;
+        const uint64_t a = buf[i], b = buf[j];
+        if ((a > b) == ((i & size) == 0u)) {
+          buf[i] = b;
+          buf[j] = a;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < limit; i += kSelThreads) out[i] = buf[i];
 }
 
 // Candidate i takes victim i when the victim is free or strictly less
@@ -487,25 +646,36 @@ meso_status meso_chunk_pool_update(meso_chunk_pool *pool, const int32_t camera_c
   hipLaunchKernelGGL(pool_begin_kernel, dim3(1), dim3(1), 0, s, pool->counters, pool->limit);
   MESO_HIP(hipGetLastError());
   const DevScene ds = pool->scene->dev();
+  // limits up to kSelMax select the first keys; larger ones sort the lists
+  const bool select = pool->limit <= kSelMax;
   if (pool->scene->kind == MESO_SCENE_BRICKMAP_U16)
     hipLaunchKernelGGL(pool_rank_kernel<2>, grid_1d((pool->cube + 255) / 256), dim3(256), 0, s,
                        ds, g, cx, cy, cz, f0, f1, f2, pool->lookup, pool->cand, pool->cube,
-                       pool->counters);
+                       pool->counters, select);
   else
     hipLaunchKernelGGL(pool_rank_kernel<3>, grid_1d((pool->cube + 255) / 256), dim3(256), 0, s,
                        ds, g, cx, cy, cz, f0, f1, f2, pool->lookup, pool->cand, pool->cube,
-                       pool->counters);
+                       pool->counters, select);
   MESO_HIP(hipGetLastError());
   const uint64_t cap = (uint64_t)g.capacity;
   hipLaunchKernelGGL(pool_victim_kernel, grid_1d((cap + 255) / 256), dim3(256), 0, s, g, cx, cy,
                      cz, f0, f1, f2, pool->slot_chunk, pool->vict);
   MESO_HIP(hipGetLastError());
-  size_t b = pool->sort_bytes;
-  MESO_HIP(rocprim::radix_sort_keys(pool->sort_tmp, b, pool->cand, pool->cand_sorted,
-                                    (size_t)pool->cube, 0, 64, s));
-  b = pool->sort_bytes;
-  MESO_HIP(rocprim::radix_sort_keys(pool->sort_tmp, b, pool->vict, pool->vict_sorted, (size_t)cap,
-                                    0, 64, s));
+  if (select) {
+    uint32_t kpad = 1u;
+    while (kpad < pool->limit) kpad <<= 1;
+    hipLaunchKernelGGL(pool_select_kernel, dim3(2), dim3(kSelThreads), 0, s, pool->cand,
+                       pool->vict, (uint32_t)cap, pool->limit, kpad, pool->counters,
+                       pool->cand_sorted, pool->vict_sorted);
+    MESO_HIP(hipGetLastError());
+  } else {
+    size_t b = pool->sort_bytes;
+    MESO_HIP(rocprim::radix_sort_keys(pool->sort_tmp, b, pool->cand, pool->cand_sorted,
+                                      (size_t)pool->cube, 0, 64, s));
+    b = pool->sort_bytes;
+    MESO_HIP(rocprim::radix_sort_keys(pool->sort_tmp, b, pool->vict, pool->vict_sorted,
+                                      (size_t)cap, 0, 64, s));
+  }
   const uint64_t lim = pool->limit;
   hipLaunchKernelGGL(pool_pair_kernel, grid_1d((lim + 255) / 256), dim3(256), 0, s,
                      pool->cand_sorted, pool->vict_sorted, pool->limit, pool->counters);

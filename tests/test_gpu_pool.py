@@ -150,3 +150,25 @@ def test_pool_argument_checks(ctx):
     pool.destroy()
     dense.destroy()
     sc.destroy()
+
+
+@pytest.mark.parametrize("capacity,radius,max_loads", [
+    (20000, 4, 64),     # more slots than the selection holds in registers
+    (5000, 9, 5000)])   # a limit above the selection's, so the lists are sorted
+def test_pool_selection_paths(ctx, capacity, radius, max_loads):
+    n, seed = 256, 5
+    sc = ctx.scene_generate(L.GEN_CAVES, L.SCENE_TWO_LEVEL_U16, n, seed)
+    osc, _, _ = _oracle_scene(L.SCENE_TWO_LEVEL_U16, L.GEN_CAVES, n, seed)
+    kw = dict(capacity=capacity, quads_per_slot=256, forward_radius=radius, backward_radius=2,
+              view_angle=100.0, max_loads=max_loads)
+    pool = R.ChunkPool(ctx, sc, mode=L.MESH_GREEDY, **kw)
+    opool = O.OraclePool(osc, O.pool_params(greedy=True, **kw))
+    for step, (cam, fwd) in enumerate(PATH[:5]):
+        pool.update(cam, fwd)
+        n_cand, nl, nev, loads = opool.update(cam, fwd)
+        st, g = _compare(pool, opool, step)
+        assert (st["n_candidates"], st["n_loads"], st["n_evicted"]) == (n_cand, nl, nev), step
+        for i, (x, y, z, sl) in enumerate(loads):
+            assert list(g["load_xyz"][3 * i:3 * i + 3]) == [x, y, z] and g["load_slot"][i] == sl
+    pool.destroy()
+    sc.destroy()
