@@ -104,27 +104,40 @@ struct SceneRows {
   int32_t ox, oy, oz;  // voxel origin of the chunk
 };
 
-template <int ES, int KIND>
-__device__ __forceinline__ Vox8<2> load8(const SceneRows<KIND> &a, int x0, int y, int z) {
-  Vox8<2> r = zero8<2>();
-  const int32_t gx = a.ox + x0, gy = a.oy + y, gz = a.oz + z;
-  const int32_t nb = a.s.n >> 3;
-  const int32_t bx = gx >> 3, by = gy >> 3, bz = gz >> 3;
-  if (bx >= nb || by >= nb || bz >= nb) return r;
-  uint32_t slot;
+// Slot of brick (bx, by, bz) of a scene, 0xFFFFFFFF when the brick is empty
+// or outside the scene.
+template <int KIND>
+__device__ __forceinline__ uint32_t scene_slot(const DevScene &s, int32_t bx, int32_t by,
+                                               int32_t bz) {
+  const int32_t nb = s.n >> 3;
+  if (bx >= nb || by >= nb || bz >= nb) return 0xFFFFFFFFu;
   if constexpr (KIND == 2) {
-    slot = a.s.slots[(uint32_t)bx + ((uint32_t)by + (uint32_t)bz * (uint32_t)nb) * (uint32_t)nb];
+    return s.slots[(uint32_t)bx + ((uint32_t)by + (uint32_t)bz * (uint32_t)nb) * (uint32_t)nb];
   } else {
-    const uint32_t nt = (uint32_t)a.s.n >> 7;
+    const uint32_t nt = (uint32_t)s.n >> 7;
     const uint32_t node =
-        a.s.top[((uint32_t)bx >> 4) + (((uint32_t)by >> 4) + ((uint32_t)bz >> 4) * nt) * nt];
-    if (node == 0xFFFFFFFFu) return r;
-    slot = a.s.slots[(uint64_t)node * 4096u + ((uint32_t)bx & 15u) + 16u * ((uint32_t)by & 15u) +
-                     256u * ((uint32_t)bz & 15u)];
+        s.top[((uint32_t)bx >> 4) + (((uint32_t)by >> 4) + ((uint32_t)bz >> 4) * nt) * nt];
+    if (node == 0xFFFFFFFFu) return 0xFFFFFFFFu;
+    return s.slots[(uint64_t)node * 4096u + ((uint32_t)bx & 15u) + 16u * ((uint32_t)by & 15u) +
+                   256u * ((uint32_t)bz & 15u)];
   }
+}
+
+// Rows of a scene chunk whose 4x4x4 brick slots are resolved: bslot[lx + 4(ly
+// + 4 lz)] for the chunk's local brick (lx, ly, lz).  Every row the mask
+// build reads lies in the chunk, so a row is one 16-byte load.
+struct SlotRows {
+  const uint32_t *bslot;
+  const uint16_t *voxels;
+};
+
+template <int ES>
+__device__ __forceinline__ Vox8<2> load8(const SlotRows &a, int x0, int y, int z) {
+  Vox8<2> r = zero8<2>();
+  const uint32_t slot = a.bslot[(x0 >> 3) + 4 * (y >> 3) + 16 * (z >> 3)];
   if (slot == 0xFFFFFFFFu) return r;
-  const uint4 v = *(const uint4 *)(a.s.voxels + (uint64_t)slot * 512u + 8u * (uint32_t)(gy & 7) +
-                                   64u * (uint32_t)(gz & 7));
+  const uint4 v = *(const uint4 *)(a.voxels + (uint64_t)slot * 512u + 8u * (uint32_t)(y & 7) +
+                                   64u * (uint32_t)(z & 7));
   r.w[0] = v.x;
   r.w[1] = v.y;
   r.w[2] = v.z;
@@ -132,9 +145,6 @@ __device__ __forceinline__ Vox8<2> load8(const SceneRows<KIND> &a, int x0, int y
   return r;
 }
 
-// flat LDS mask planes: O, Ex, Ey, Ez ([z][y], bits over x, row stride 33 so
-// that the 32 lanes reading row v of 32 different z planes hit 32 different
-// banks), OT, EyT, EzT ([z][x], bits over y, stride 32)
 constexpr int kZY = 33;
 enum : int {
   kO = 0,
@@ -152,6 +162,7 @@ struct MeshLds {
   uint32_t pex[kPlanes];  // culled: exclusive quad offset of each plane in the chunk
   uint32_t occ[kPlanes];  // culled: bit v set when row v of the plane has a face
   uint32_t scan[kMeshBlock / 64];
+  uint32_t bslot[64];  // scene chunks: slots of the chunk's 4x4x4 bricks
   uint32_t chunk;
   uint32_t base_lo, base_hi;
 };
@@ -186,7 +197,8 @@ __device__ __forceinline__ void build_masks(MeshLds &L, const Rows &a) {
   constexpr uint32_t nwaves = kMeshBlock / 64;
   const int k = (int)(lane & 3u);
 #pragma unroll 2
-  for (uint32_t blk = wave; blk < 64u; blk += nwaves) {
+  for (uint32_t it = 0; it < 64u / nwaves; it++) {
+    const uint32_t blk = wave + it * nwaves;
     const uint32_t r = blk * 16u + (lane >> 2);
     const int y = (int)(r & 31u), z = (int)(r >> 5);
     const Vox8<ES> c = load8<ES>(a, 8 * k, y, z);
@@ -232,6 +244,20 @@ __device__ __forceinline__ void build_masks(MeshLds &L, const Rows &a) {
     }
   }
   __syncthreads();
+}
+
+// Masks of scene chunk (ox, oy, oz) / 32: the slots of its 64 bricks are
+// resolved first, one lane each, so that the row loads do not wait on the
+// brick lookups.
+template <int ES, int KIND>
+__device__ __forceinline__ void build_masks(MeshLds &L, const SceneRows<KIND> &a) {
+  const uint32_t t = threadIdx.x;
+  if (t < 64u)
+    L.bslot[t] = scene_slot<KIND>(a.s, (a.ox >> 3) + (int32_t)(t & 3u),
+                                  (a.oy >> 3) + (int32_t)((t >> 2) & 3u),
+                                  (a.oz >> 3) + (int32_t)(t >> 4));
+  __syncthreads();
+  build_masks<ES>(L, SlotRows{L.bslot, a.s.voxels});
 }
 
 // Offsets of plane (f, s) in the mask planes.  z faces: (u,v) = (x,y) and

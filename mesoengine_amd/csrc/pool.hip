@@ -236,16 +236,17 @@ __global__ void __launch_bounds__(kSelThreads)
       each([&](uint64_t key, bool valid) {
         const bool m = valid && (key & hi) == prefix;
         const uint32_t d = (uint32_t)(key >> shift) & 255u;
-        const uint64_t mb = __ballot(m);
-        if (!mb) return;
-        // one atomic for a wave whose matching keys share the digit
-        const uint32_t first = (uint32_t)__ffsll((long long)mb) - 1u;
-        const uint32_t d0 = __shfl(d, (int)first);
-        if (__ballot(m && d == d0) == mb) {
-          if (lane == first) atomicAdd(&hist[d0], (uint32_t)__popcll(mb));
-        } else if (m) {
-          atomicAdd(&hist[d], 1u);
+        // the wave's first four distinct digits take one atomic each (the
+        // lanes of a digit leave the set together); the rest add per lane
+        uint64_t left = __ballot(m);
+        for (int round = 0; round < 4 && left; round++) {
+          const uint32_t first = (uint32_t)__ffsll((long long)left) - 1u;
+          const uint32_t d0 = __shfl(d, (int)first);
+          const uint64_t same = __ballot(m && d == d0) & left;
+          if (lane == first) atomicAdd(&hist[d0], (uint32_t)__popcll(same));
+          left &= ~same;
         }
+        if ((left >> lane) & 1ull) atomicAdd(&hist[d], 1u);
       });
       __syncthreads();
       if (tid < 64u) {
