@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(256) pool_rank_kernel(DevScene s, PoolGeom g, 
   uint32_t base = 0u;
   if (lane == 0u && b) base = atomicAdd(&counters[kCntCandidates], (uint32_t)__popcll(b));
   if (compact) {
-    base = __shfl(base, 0);
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
     if (cand) keys[base + (uint32_t)__popcll(b & ((1ull << lane) - 1ull))] = key;
   }
 }
@@ -200,6 +200,7 @@ __global__ void __launch_bounds__(kSelThreads)
                        uint64_t *vict_out) {
   __shared__ uint64_t buf[kSelMax];
   __shared__ uint32_t hist[256];
+  __shared__ uint32_t spread_hist[16 * 257];  // 16 copies, by lane, for the top two digits
   __shared__ uint32_t s_bin, s_below, s_cnt, s_fill;
   __shared__ uint64_t s_key;
   const uint32_t tid = threadIdx.x, lane = tid & 63u;
@@ -231,24 +232,29 @@ __global__ void __launch_bounds__(kSelThreads)
     int shift = 56;
     bool alone = false;
     for (;; shift -= 8) {
+      // The top digits are shared by most keys of a wave; lanes that add to
+      // one LDS word serialize, so those passes count into 16 copies
+      // (lane & 15, stride 257 words) that are summed afterwards.
+      const bool spread = shift >= 48;
       if (tid < 256u) hist[tid] = 0u;
+      if (spread)
+        for (uint32_t i = tid; i < 16u * 257u; i += kSelThreads) spread_hist[i] = 0u;
       __syncthreads();
       each([&](uint64_t key, bool valid) {
         const bool m = valid && (key & hi) == prefix;
         const uint32_t d = (uint32_t)(key >> shift) & 255u;
-        // the wave's first four distinct digits take one atomic each (the
-        // lanes of a digit leave the set together); the rest add per lane
-        uint64_t left = __ballot(m);
-        for (int round = 0; round < 4 && left; round++) {
-          const uint32_t first = (uint32_t)__ffsll((long long)left) - 1u;
-          const uint32_t d0 = __shfl(d, (int)first);
-          const uint64_t same = __ballot(m && d == d0) & left;
-          if (lane == first) atomicAdd(&hist[d0], (uint32_t)__popcll(same));
-          left &= ~same;
-        }
-        if ((left >> lane) & 1ull) atomicAdd(&hist[d], 1u);
+        if (m) atomicAdd(spread ? &spread_hist[(lane & 15u) * 257u + d] : &hist[d], 1u);
       });
       __syncthreads();
+      if (spread) {
+        if (tid < 256u) {
+          uint32_t acc = 0u;
+#pragma unroll
+          for (uint32_t c = 0; c < 16u; c++) acc += spread_hist[c * 257u + tid];
+          hist[tid] = acc;
+        }
+        __syncthreads();
+      }
       if (tid < 64u) {
         const uint32_t h0 = hist[4u * tid], h1 = hist[4u * tid + 1u], h2 = hist[4u * tid + 2u],
                        h3 = hist[4u * tid + 3u];
@@ -308,7 +314,7 @@ __global__ void __launch_bounds__(kSelThreads)
     if (!mb) return;
     uint32_t base = 0u;
     if (lane == 0u) base = atomicAdd(&s_fill, (uint32_t)__popcll(mb));
-    base = __shfl(base, 0);
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
     const uint32_t pos = base + (uint32_t)__popcll(mb & ((1ull << lane) - 1ull));
     if (m && pos < below) buf[pos] = key;
   });
