@@ -184,7 +184,8 @@ __global__ void __launch_bounds__(256) pool_victim_kernel(PoolGeom g, int32_t cx
 // number is the candidate counter), block 1 the victims (one per slot).
 // An MSB-first radix select over 8-bit digits finds t, the limit-th
 // smallest key: per digit a histogram in LDS of the keys that share the
-// digits found so far, and a scan by one wave.  The keys below t are
+// digits found so far, and a scan by one wave; once at most kSelSmall keys
+// share them, one wave ranks those keys directly.  The keys below t are
 // gathered into LDS, the remaining places up to the limit take t, and a
 // bitonic sort orders the kpad (power of two >= limit) places.  That is
 // the first `limit` places of a full sort of the list, repeated keys
@@ -192,7 +193,7 @@ __global__ void __launch_bounds__(256) pool_victim_kernel(PoolGeom g, int32_t cx
 // with the largest key, which is what a full sort of the cube's keys puts
 // after the candidates.  The first kSelPer * kSelThreads keys stay in
 // registers across the passes; the rest are re-read from memory.
-constexpr uint32_t kSelThreads = 1024u, kSelMax = 4096u, kSelPer = 16u;
+constexpr uint32_t kSelThreads = 1024u, kSelMax = 4096u, kSelPer = 16u, kSelSmall = 64u;
 
 __global__ void __launch_bounds__(kSelThreads)
     pool_select_kernel(const uint64_t *cand, const uint64_t *vict, uint32_t cap, uint32_t limit,
@@ -227,10 +228,11 @@ __global__ void __launch_bounds__(kSelThreads)
   };
   uint64_t t = ~0ull;     // the limit-th smallest key
   uint32_t rem = limit;   // its rank among the keys sharing the digits found
+  uint32_t less = 0u;     // keys sharing them that are below t
   if (n > limit) {
     uint64_t prefix = 0ull, hi = 0ull;  // digits found, and their bits
     int shift = 56;
-    bool alone = false;
+    bool small = false;
     for (;; shift -= 8) {
       // The top digits are shared by most keys of a wave; lanes that add to
       // one LDS word serialize, so those passes count into 16 copies
@@ -286,25 +288,49 @@ __global__ void __launch_bounds__(kSelThreads)
       prefix |= (uint64_t)s_bin << shift;
       hi |= 255ull << shift;
       rem -= s_below;
-      if (shift == 0) break;
-      // one key holds the digits found: it is t
-      if (s_cnt == 1u) {
-        alone = true;
+      if (s_cnt <= kSelSmall) {
+        small = true;
         break;
       }
+      if (shift == 0) break;
     }
-    if (alone) {
+    if (small) {
+      // few keys hold the digits found: gather them, and one wave ranks
+      // them (ties by place); the key of rank rem - 1 is t
+      if (tid == 0u) s_fill = 0u;
+      __syncthreads();
       each([&](uint64_t key, bool valid) {
-        if (valid && (key & hi) == prefix) s_key = key;
+        if (valid && (key & hi) == prefix) {
+          const uint32_t pos = atomicAdd(&s_fill, 1u);
+          if (pos < kSelSmall) buf[pos] = key;
+        }
       });
       __syncthreads();
+      const uint32_t cnt = s_cnt;
+      if (tid < cnt) {
+        const uint64_t k = buf[tid];
+        uint32_t rank = 0u, lt = 0u;
+        for (uint32_t j = 0; j < cnt; j++) {
+          const uint64_t o = buf[j];
+          lt += o < k ? 1u : 0u;
+          rank += (o < k || (o == k && j < tid)) ? 1u : 0u;
+        }
+        if (rank == rem - 1u) {
+          s_key = k;
+          s_fill = lt;
+        }
+      }
+      __syncthreads();
       t = s_key;
+      less = s_fill;
+      __syncthreads();  // s_fill is reset for the gather below
     } else {
-      t = prefix;
+      t = prefix;  // every key holding all eight digits equals t
     }
   }
-  // keys below t: n when the list is taken whole, else limit - rem
-  const uint32_t below = n > limit ? limit - rem : n;
+  // keys below t: n when the list is taken whole, else the keys under the
+  // digits' bin plus those of the small set below t
+  const uint32_t below = n > limit ? limit - rem + less : n;
   if (tid == 0u) s_fill = 0u;
   for (uint32_t i = below + tid; i < kpad; i += kSelThreads) buf[i] = i < limit ? t : ~0ull;
   __syncthreads();
@@ -318,16 +344,45 @@ __global__ void __launch_bounds__(kSelThreads)
     const uint32_t pos = base + (uint32_t)__popcll(mb & ((1ull << lane) - 1ull));
     if (m && pos < below) buf[pos] = key;
   });
+  // bitonic sort, two layers (strides s and s/2) per barrier where the
+  // stage has them: a thread orders the four places i, i + s/2, i + s,
+  // i + 3s/2 in registers
   for (uint32_t size = 2u; size <= kpad; size <<= 1) {
-    for (uint32_t stride = size >> 1; stride > 0u; stride >>= 1) {
+    for (uint32_t stride = size >> 1; stride > 0u;) {
       __syncthreads();
-      for (uint32_t q = tid; q < kpad / 2u; q += kSelThreads) {
-        const uint32_t i = 2u * q - (q & (stride - 1u)), j = i + stride;
-        const uint64_t a = buf[i], b = buf[j];
-        if ((a > b) == ((i & size) == 0u)) {
-          buf[i] = b;
-          buf[j] = a;
+      if (stride >= 2u) {
+        const uint32_t h = stride >> 1, p1 = (uint32_t)__ffs((int)h) - 1u;
+        for (uint32_t q = tid; q < kpad / 4u; q += kSelThreads) {
+          const uint32_t i = ((q >> p1) << (p1 + 2u)) | (q & (h - 1u));
+          const bool up = (i & size) == 0u;
+          uint64_t a0 = buf[i], a1 = buf[i + h], a2 = buf[i + stride], a3 = buf[i + stride + h];
+          auto order = [up](uint64_t &x, uint64_t &y) {
+            if ((x > y) == up) {
+              const uint64_t z = x;
+              x = y;
+              y = z;
+            }
+          };
+          order(a0, a2);
+          order(a1, a3);
+          order(a0, a1);
+          order(a2, a3);
+          buf[i] = a0;
+          buf[i + h] = a1;
+          buf[i + stride] = a2;
+          buf[i + stride + h] = a3;
         }
+        stride >>= 2;
+      } else {
+        for (uint32_t q = tid; q < kpad / 2u; q += kSelThreads) {
+          const uint32_t i = 2u * q;
+          const uint64_t a = buf[i], b = buf[i + 1u];
+          if ((a > b) == ((i & size) == 0u)) {
+            buf[i] = b;
+            buf[i + 1u] = a;
+          }
+        }
+        stride = 0u;
       }
     }
   }
