@@ -27,6 +27,8 @@
 //     its output offset by an ordered decoupled look-back over the chunks
 //     ticketed before it, so one persistent launch writes one compact quad
 //     buffer plus per-chunk counts and exclusive prefixes.
+#include <type_traits>
+
 #include "meso_internal.h"
 
 namespace meso {
@@ -196,36 +198,49 @@ __device__ __forceinline__ void build_masks(MeshLds &L, const Rows &a) {
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   constexpr uint32_t nwaves = kMeshBlock / 64;
   const int k = (int)(lane & 3u);
-#pragma unroll 2
-  for (uint32_t it = 0; it < 64u / nwaves; it++) {
-    const uint32_t blk = wave + it * nwaves;
-    const uint32_t r = blk * 16u + (lane >> 2);
-    const int y = (int)(r & 31u), z = (int)(r >> 5);
-    const Vox8<ES> c = load8<ES>(a, 8 * k, y, z);
-    const Vox8<ES> cy = y < 31 ? load8<ES>(a, 8 * k, y + 1, z) : zero8<ES>();
-    const Vox8<ES> cz = z < 31 ? load8<ES>(a, 8 * k, y, z + 1) : zero8<ES>();
-    const uint32_t next0 = __shfl_down(c.w[0], 1, 64);  // first voxels of x + 8
-    uint32_t o = 0, ex = 0, ey = 0, ez = 0;
+  // two row blocks per trip: all their loads are issued before either
+  // block's arithmetic, so a trip waits on memory once
+  // (scene chunks only: the batched packed-chunk meshers hide latency with
+  // occupancy, and measured slower with the extra registers)
+  constexpr uint32_t kTrips = 64u / nwaves, kPer = std::is_same<Rows, SlotRows>::value ? 2u : 1u;
+  for (uint32_t it = 0; it < kTrips; it += kPer) {
+    Vox8<ES> c[kPer], cy[kPer], cz[kPer];
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const uint32_t v = c.get(j);
-      o |= (uint32_t)(v != 0u) << j;
-      const uint32_t vx = j < 7 ? c.get(j + 1) : (ES == 1 ? next0 & 0xFFu : next0 & 0xFFFFu);
-      ex |= (uint32_t)(v == vx && (j < 7 || k < 3)) << j;
-      ey |= (uint32_t)(y < 31 && v == cy.get(j)) << j;
-      ez |= (uint32_t)(z < 31 && v == cz.get(j)) << j;
+    for (uint32_t b = 0; b < kPer; b++) {
+      const uint32_t r = (wave + (it + b) * nwaves) * 16u + (lane >> 2);
+      const int y = (int)(r & 31u), z = (int)(r >> 5);
+      c[b] = load8<ES>(a, 8 * k, y, z);
+      cy[b] = y < 31 ? load8<ES>(a, 8 * k, y + 1, z) : zero8<ES>();
+      cz[b] = z < 31 ? load8<ES>(a, 8 * k, y, z + 1) : zero8<ES>();
     }
-    const int sh = 8 * k;
-    o = row_or(o << sh);
-    ex = row_or(ex << sh);
-    ey = row_or(ey << sh);
-    ez = row_or(ez << sh);
-    if (k == 0) {
-      const int zy = z * kZY + y;
-      L.M[kO + zy] = o;
-      L.M[kEx + zy] = ex;
-      L.M[kEy + zy] = ey;
-      L.M[kEz + zy] = ez;
+#pragma unroll
+    for (uint32_t b = 0; b < kPer; b++) {
+      const uint32_t r = (wave + (it + b) * nwaves) * 16u + (lane >> 2);
+      const int y = (int)(r & 31u), z = (int)(r >> 5);
+      const uint32_t next0 = __shfl_down(c[b].w[0], 1, 64);  // first voxels of x + 8
+      uint32_t o = 0, ex = 0, ey = 0, ez = 0;
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const uint32_t v = c[b].get(j);
+        o |= (uint32_t)(v != 0u) << j;
+        const uint32_t vx =
+            j < 7 ? c[b].get(j + 1) : (ES == 1 ? next0 & 0xFFu : next0 & 0xFFFFu);
+        ex |= (uint32_t)(v == vx && (j < 7 || k < 3)) << j;
+        ey |= (uint32_t)(y < 31 && v == cy[b].get(j)) << j;
+        ez |= (uint32_t)(z < 31 && v == cz[b].get(j)) << j;
+      }
+      const int sh = 8 * k;
+      o = row_or(o << sh);
+      ex = row_or(ex << sh);
+      ey = row_or(ey << sh);
+      ez = row_or(ez << sh);
+      if (k == 0) {
+        const int zy = z * kZY + y;
+        L.M[kO + zy] = o;
+        L.M[kEx + zy] = ex;
+        L.M[kEy + zy] = ey;
+        L.M[kEz + zy] = ez;
+      }
     }
   }
   __syncthreads();
