@@ -154,6 +154,7 @@ def test_pool_argument_checks(ctx):
 
 @pytest.mark.parametrize("capacity,radius,max_loads", [
     (20000, 4, 64),     # more slots than the selection holds in registers
+    (5000, 9, 4096),    # the selection's largest limit: 4,096 places sorted in LDS
     (5000, 9, 5000)])   # a limit above the selection's, so the lists are sorted
 def test_pool_selection_paths(ctx, capacity, radius, max_loads):
     n, seed = 256, 5
