@@ -517,17 +517,24 @@ typedef struct {
   uint32_t size;
   uint64_t hbm_budget_bytes;  /* bytes of request outputs admitted at once */
   int32_t max_pending;        /* queued, not yet admitted; submit -> BUSY beyond */
-  int32_t max_inflight;       /* launches in flight (1..4) */
-  int32_t rays_per_launch;    /* work bound of one batched launch */
-  int32_t tile;               /* pixel tile edge of a work item (8 or 16) */
+  int32_t max_inflight;       /* admitted requests on the device at once: entries of the
+                                 device ring the persistent kernel draws from (1..4096) */
+  int32_t workgroups_per_cu;  /* persistent workgroups of 256 lanes per CU (0: all that fit) */
+  int32_t tile;               /* pixel tile edge of a work item (0 or 16) */
+  int32_t idle_exit_us;       /* the persistent kernel returns after this long with nothing
+                                 to claim, and is launched again for new work (0: 500) */
 } meso_serve_config;
 
 typedef struct {
   uint32_t size;
   uint64_t submitted, admitted, completed, refused_busy;
-  uint64_t launches, rays, shadow_rays;
+  uint64_t launches;          /* launches of the persistent kernel */
+  uint64_t primary_rays;      /* pixels of the admitted requests */
   uint64_t bytes_in_use, peak_bytes_in_use;
-  double busy_ms;             /* sum of launch durations */
+  double busy_ms;             /* summed lifetimes of those launches */
+  double queue_us_sum, queue_us_max;  /* submit -> admission */
+  uint64_t queued_over_1ms;   /* requests that waited more than 1 ms for admission */
+  uint64_t admission_blocked_passes;  /* scheduler passes that left requests waiting */
 } meso_serve_stats;
 
 typedef struct {

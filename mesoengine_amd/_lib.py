@@ -74,15 +74,18 @@ class TuneResult(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("size", C.c_uint32), ("hbm_budget_bytes", C.c_uint64),
                 ("max_pending", C.c_int32), ("max_inflight", C.c_int32),
-                ("rays_per_launch", C.c_int32), ("tile", C.c_int32)]
+                ("workgroups_per_cu", C.c_int32), ("tile", C.c_int32),
+                ("idle_exit_us", C.c_int32)]
 
 
 class ServeStats(C.Structure):
     _fields_ = [("size", C.c_uint32), ("submitted", C.c_uint64), ("admitted", C.c_uint64),
                 ("completed", C.c_uint64), ("refused_busy", C.c_uint64),
-                ("launches", C.c_uint64), ("rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("launches", C.c_uint64), ("primary_rays", C.c_uint64),
                 ("bytes_in_use", C.c_uint64), ("peak_bytes_in_use", C.c_uint64),
-                ("busy_ms", C.c_double)]
+                ("busy_ms", C.c_double), ("queue_us_sum", C.c_double),
+                ("queue_us_max", C.c_double), ("queued_over_1ms", C.c_uint64),
+                ("admission_blocked_passes", C.c_uint64)]
 
 
 class HostOutputs(C.Structure):

@@ -194,6 +194,10 @@ __global__ void __launch_bounds__(256) pool_victim_kernel(PoolGeom g, int32_t cx
 // after the candidates.  The first kSelPer * kSelThreads keys stay in
 // registers across the passes; the rest are re-read from memory.
 constexpr uint32_t kSelThreads = 1024u, kSelMax = 4096u, kSelPer = 16u, kSelSmall = 64u;
+// Longest list the one-workgroup select takes: keys past the first
+// kSelPer * kSelThreads are re-read on every radix pass, so past eight times
+// that the device-wide radix sorts are faster.
+constexpr uint64_t kSelListMax = 8ull * kSelPer * kSelThreads;
 
 __global__ void __launch_bounds__(kSelThreads)
     pool_select_kernel(const uint64_t *cand, const uint64_t *vict, uint32_t cap, uint32_t limit,
@@ -510,25 +514,29 @@ struct meso_chunk_pool {
   std::vector<float> dirs;  // baked view directions, 3 per direction (empty: exact)
 };
 
-// unit vector v / length, length = sqrt((x*x + y*y) + z*z), in float
-static void normalize_f(const float v[3], float out[3]) {
-  float ss = v[0] * v[0] + v[1] * v[1];
-  ss = ss + v[2] * v[2];
-  const float len = sqrtf(ss);
-  for (int k = 0; k < 3; k++) out[k] = v[k] / len;
-}
-
-// GetFibonacciSphere (Runtimes/Helper/VoxelMathHelper.h:48-71): y = 1 - 2i/(n-1),
-// radius sqrt(1 - y^2), angle i * pi * (sqrt(5) - 1), in double, then float
+// GetFibonacciSphere<float> (Runtimes/Helper/VoxelMathHelper.h:47-68) as
+// BakeVisibilityByView instantiates it (ChunkManagerHelper.h:205): Phi in
+// double stored as float, then float arithmetic throughout (cosf, sinf,
+// sqrtf), and glm::normalize's v * (1 / sqrt(dot(v, v))).  Host code, so the
+// directions are the same bits as any other host evaluation on this libm.
 static void bake_directions(int32_t n, std::vector<float> &dirs) {
   dirs.resize((size_t)n * 3);
-  const double phi = 3.14159265358979323846 * (std::sqrt(5.0) - 1.0);
+  const float phi = (float)(3.14159265358979323846 * (std::sqrt(5.0) - 1.0));
   for (int32_t i = 0; i < n; i++) {
-    const double y = 1.0 - ((double)i / (double)(n - 1)) * 2.0;
-    const double r = std::sqrt(1.0 - y * y);
-    const double th = phi * (double)i;
-    const float v[3] = {(float)(std::cos(th) * r), (float)y, (float)(std::sin(th) * r)};
-    normalize_f(v, &dirs[(size_t)i * 3]);
+    float y = (float)i / (float)(n - 1);
+    y = y * 2.0f;
+    y = 1.0f - y;
+    const float yy = y * y;
+    const float r = sqrtf(1.0f - yy);
+    const float th = phi * (float)i;
+    const float x = cosf(th) * r;
+    const float z = sinf(th) * r;
+    float dd = x * x + yy;
+    dd = dd + z * z;
+    const float inv = 1.0f / sqrtf(dd);
+    dirs[(size_t)i * 3 + 0] = x * inv;
+    dirs[(size_t)i * 3 + 1] = y * inv;
+    dirs[(size_t)i * 3 + 2] = z * inv;
   }
 }
 
@@ -708,8 +716,11 @@ meso_status meso_chunk_pool_update(meso_chunk_pool *pool, const int32_t camera_c
   hipLaunchKernelGGL(pool_begin_kernel, dim3(1), dim3(1), 0, s, pool->counters, pool->limit);
   MESO_HIP(hipGetLastError());
   const DevScene ds = pool->scene->dev();
-  // limits up to kSelMax select the first keys; larger ones sort the lists
-  const bool select = pool->limit <= kSelMax;
+  // limits up to kSelMax select the first keys of lists up to kSelListMax
+  // keys (candidates: at most one per cube offset; victims: one per slot);
+  // larger limits or longer lists sort the lists
+  const bool select = pool->limit <= kSelMax && (uint64_t)pool->cube <= kSelListMax &&
+                      (uint64_t)pool->geom.capacity <= kSelListMax;
   if (pool->scene->kind == MESO_SCENE_BRICKMAP_U16)
     hipLaunchKernelGGL(pool_rank_kernel<2>, grid_1d((pool->cube + 255) / 256), dim3(256), 0, s,
                        ds, g, cx, cy, cz, f0, f1, f2, pool->lookup, pool->cand, pool->cube,

@@ -9,7 +9,7 @@ namespace meso_engine {
 // ------------------------------------------------------------------ camera
 void FVoxelCamera::SetPose(const FVec3 &world_eye, const FVec3 &forward,
                            const FVoxelSceneConfig &cfg) {
-  const float cs = cfg.GetChunkSize();
+  const float cs = cfg.GetMeshChunkSize();
   // floating origin: whole chunks in CameraChunkLocation, the rest in Position
   // (x - floor(x / cs) * cs is exact in float for a power-of-two chunk size)
   CameraChunkLocation = {(int32_t)std::floor(world_eye.x / cs), (int32_t)std::floor(world_eye.y / cs),
@@ -21,13 +21,13 @@ void FVoxelCamera::SetPose(const FVec3 &world_eye, const FVec3 &forward,
 }
 
 FVec3 FVoxelCamera::GetWorldPosition(const FVoxelSceneConfig &cfg) const {
-  const float cs = cfg.GetChunkSize();
+  const float cs = cfg.GetMeshChunkSize();
   return {(float)CameraChunkLocation.x * cs + Position.x, (float)CameraChunkLocation.y * cs + Position.y,
           (float)CameraChunkLocation.z * cs + Position.z};
 }
 
 void FVoxelCamera::UpdateCamera(const FVoxelSceneConfig &cfg) {
-  const float cs = cfg.GetChunkSize();
+  const float cs = cfg.GetMeshChunkSize();
   // ConvertToChunkLocation (Runtimes/Helper/VoxelMathHelper.h:17-22): whole
   // chunks of the in-chunk position move into CameraChunkLocation
   const FIVec3 off{(int32_t)std::floor(Position.x / cs), (int32_t)std::floor(Position.y / cs),
@@ -93,6 +93,12 @@ bool FHipVoxelInstance::Check(meso_status st) {
 }
 
 meso_status FHipVoxelInstance::Initialize(int32_t device) {
+  if (VoxelSceneConfig.MeshChunkResolution != 32 || !(VoxelSceneConfig.BlockSize > 0.0f)) {
+    // edits, remeshing and the pool work on the library's 32^3 chunks
+    status_ = MESO_ERR_ARGUMENT_OUT_OF_RANGE;
+    error_ = "FVoxelSceneConfig: MeshChunkResolution must be 32 and BlockSize positive";
+    return status_;
+  }
   if (!Check(meso_context_create(device, &ctx_))) return status_;
   InitializeBegin();
   return status_;

@@ -31,24 +31,57 @@ struct FIVec3 {
   bool operator!=(const FIVec3 &o) const { return !(*this == o); }
 };
 
-// The scene geometry the path uses (Runtimes/Voxel/VoxelSceneConfig.h:20-49).
-// ChunkResolution is the edge, in voxels, of the chunks that edits, remeshing,
-// streaming and the camera's chunk tracking work on: 32 here (the reference's
-// chunks are 16 blocks of 8^3).  The streaming fields keep the reference's
-// names and defaults; ChunkQuadCapacity (quads a pool slot holds) is new.
+// How a full pool picks a slot to override (Runtimes/Voxel/VoxelSceneConfig.h:12-17).
+// The chunk pool on the GPU implements FindMin: a candidate replaces the
+// least important resident chunk when it is more important.
+enum class EChunkOverrideMode : uint8_t {
+  FindLess = 1 << 0,
+  FindMin = 1 << 1,
+  OverrideMin = 1 << 2,
+};
+
+// The reference's scene configuration (Runtimes/Voxel/VoxelSceneConfig.h:20-49):
+// every field, in the same order and with the same default, so the
+// reference's initialisers (designated or member by member) compile
+// unchanged.  GetChunkSize() keeps its meaning: ChunkResolution blocks of
+// BlockSize.  Fields after the reference's are this engine's own:
+// MeshChunkResolution is the edge, in voxels, of the chunks that edits,
+// remeshing, streaming and the camera's chunk tracking work on (32: a mesh
+// chunk's rows are 32-bit masks), and ChunkQuadCapacity the quads one pool
+// slot holds.  The pool reads MaxChunkCount, the two load radii,
+// ViewChunkAngle, the two load counts and BakeVisibilityViewNum; the other
+// reference fields size CPU pools, thread batches and erosion depths that
+// the GPU path does not have, and are kept for source compatibility.
 struct FVoxelSceneConfig {
   unsigned char BlockResolution = 8;
   float BlockSize = 1.0f;
-  unsigned char ChunkResolution = 32;
+  unsigned char ChunkResolution = 16;
+  uint32_t MaxBlockCount = 65536 * 16;
+  uint32_t MaxVolumeCount = 65536 * 16;
   uint32_t MaxChunkCount = 8192 * 2;
+  uint32_t MaxEmptyChunkCount = 8192 * 4;
+
+  uint32_t MaxChunkCheckTimes = 128;
+  uint32_t MaxEmptyChunkCheckTimes = 128;
+  uint32_t MaxBlockCheckTimes = 16;
+
+  uint32_t BakeVisibilityViewNum = 256;
   uint32_t ViewForwardLoadChunkSize = 24;
   uint32_t ViewBackwardLoadChunkSize = 6;
   uint32_t MaxSyncedLoadChunkCount = 0;
   uint32_t MaxUnsyncedLoadChunkCount = 256;
+  uint32_t ChunkTaskPerCore = 8;
+  EChunkOverrideMode ChunkOverrideMode = EChunkOverrideMode::FindMin;
   float ViewChunkAngle = 120.0f;
-  uint32_t BakeVisibilityViewNum = 256;
-  uint32_t ChunkQuadCapacity = 8192;
+
+  uint32_t ChunkOccupancyDepth = 4;
+  uint32_t ChunkInnerVoxelCullDepthThreshold = 1;
   float GetChunkSize() const { return (float)ChunkResolution * BlockSize; }
+
+  // this engine's fields
+  uint32_t MeshChunkResolution = 32;
+  uint32_t ChunkQuadCapacity = 8192;
+  float GetMeshChunkSize() const { return (float)MeshChunkResolution * BlockSize; }
 };
 
 // Camera with FVoxelCamera's floating origin (Runtimes/Instance/VoxelCamera.cpp:42-59):

@@ -279,7 +279,6 @@ int main(int argc, char **argv) {
   d.max_dirty_chunks = dirty > 0 ? dirty : 1;
 
   FHipVoxelInstance inst(d);
-  inst.VoxelSceneConfig.ChunkResolution = 32;
   if (pool_capacity > 0) inst.VoxelSceneConfig.MaxChunkCount = (uint32_t)pool_capacity;
   if (pool_radius > 0) inst.VoxelSceneConfig.ViewForwardLoadChunkSize = (uint32_t)pool_radius;
   if (pool_loads > 0) inst.VoxelSceneConfig.MaxUnsyncedLoadChunkCount = (uint32_t)pool_loads;

@@ -114,6 +114,7 @@ struct Result {
   std::vector<double> lat_ms;
   double makespan_s = 0.0;
   uint64_t launches = 0;
+  meso_serve_stats stats{};  // served path only
 };
 
 // plain path: one raymarch per request on the context stream, in arrival order
@@ -218,13 +219,14 @@ Result run_served(meso_context *ctx, std::vector<meso_scene *> &scenes, const st
   }
   res.lat_ms = lat_ms;
   res.launches = stats.launches;
+  res.stats = stats;
   return res;
 }
 
 int usage() {
   std::fprintf(stderr,
                "usage: meso_serve_bench REQUESTS [--budget-mb M] [--inflight K]\n"
-               "  [--rays-per-launch R] [--capacity] [--device I]\n");
+               "  [--wg-per-cu W] [--capacity] [--device I]\n");
   return 2;
 }
 
@@ -233,7 +235,7 @@ int usage() {
 int main(int argc, char **argv) {
   if (argc < 2) return usage();
   uint64_t budget_mb = 512;
-  int32_t inflight = 2, rays_per_launch = 1 << 20, device = 0;
+  int32_t inflight = 64, wg_per_cu = 0, device = 0;
   bool capacity = false;
   for (int i = 2; i < argc; i++) {
     const std::string a = argv[i];
@@ -245,7 +247,7 @@ int main(int argc, char **argv) {
     const char *v = argv[++i];
     if (a == "--budget-mb") budget_mb = std::strtoull(v, nullptr, 10);
     else if (a == "--inflight") inflight = std::atoi(v);
-    else if (a == "--rays-per-launch") rays_per_launch = std::atoi(v);
+    else if (a == "--wg-per-cu") wg_per_cu = std::atoi(v);
     else if (a == "--device") device = std::atoi(v);
     else return usage();
   }
@@ -300,7 +302,7 @@ int main(int argc, char **argv) {
   cfg.hbm_budget_bytes = budget_mb << 20;
   cfg.max_pending = (int32_t)reqs.size() + 8;
   cfg.max_inflight = inflight;
-  cfg.rays_per_launch = rays_per_launch;
+  cfg.workgroups_per_cu = wg_per_cu;
   cfg.tile = 16;
   const bool timed = !capacity;
   // warm both paths once on the first few requests
@@ -317,14 +319,17 @@ int main(int argc, char **argv) {
               "\"plain\": {\"p50_ms\": %.3f, \"p99_ms\": %.3f, \"max_ms\": %.3f, \"makespan_s\": %.4f, "
               "\"mrays\": %.1f}, "
               "\"served\": {\"p50_ms\": %.3f, \"p99_ms\": %.3f, \"max_ms\": %.3f, \"makespan_s\": %.4f, "
-              "\"mrays\": %.1f, \"launches\": %llu}}\n",
+              "\"mrays\": %.1f, \"launches\": %llu, \"peak_budget_mb\": %.1f, "
+              "\"queue_ms_mean\": %.3f, \"queue_ms_max\": %.3f, \"queued_over_1ms\": %llu}}\n",
               reqs.size(), scenes.size(), resident / 1e6, (unsigned long long)rays,
               capacity ? "capacity" : "timed", span > 0 ? (double)reqs.size() / span : 0.0,
               (unsigned long long)budget_mb, pct(plain.lat_ms, 50), pct(plain.lat_ms, 99),
               pct(plain.lat_ms, 100), plain.makespan_s, rays / plain.makespan_s / 1e6,
               pct(served.lat_ms, 50), pct(served.lat_ms, 99), pct(served.lat_ms, 100),
               served.makespan_s, rays / served.makespan_s / 1e6,
-              (unsigned long long)served.launches);
+              (unsigned long long)served.launches, served.stats.peak_bytes_in_use / 1048576.0,
+              served.stats.admitted ? served.stats.queue_us_sum / served.stats.admitted * 1e-3 : 0.0,
+              served.stats.queue_us_max * 1e-3, (unsigned long long)served.stats.queued_over_1ms);
   for (meso_scene *s : scenes) meso_scene_destroy(s);
   meso_context_destroy(ctx);
   return 0;

@@ -119,6 +119,10 @@ class Buffer:
     def zero(self, stream=None):
         check(self.lib.meso_buffer_fill_zero(self.h, stream))
 
+    def fill(self, byte):
+        """every byte set to `byte`, by a synchronous upload"""
+        self.upload(np.full(self.nbytes, byte, np.uint8))
+
     def free(self):
         if self.h:
             self.lib.meso_buffer_destroy(self.h)
@@ -284,11 +288,15 @@ class Event:
 
 
 class Server:
-    def __init__(self, ctx, budget_bytes, max_pending=256, max_inflight=2,
-                 rays_per_launch=1 << 20):
+    """The continuous-batching server: requests are admitted by the HBM
+    budget into `max_inflight` ring entries that one persistent kernel of
+    `workgroups_per_cu` workgroups per CU (0: all that fit) draws tiles from."""
+
+    def __init__(self, ctx, budget_bytes, max_pending=256, max_inflight=64, workgroups_per_cu=0,
+                 idle_exit_us=0):
         self.ctx = ctx
         cfg = L.ServeConfig(C.sizeof(L.ServeConfig), budget_bytes, max_pending, max_inflight,
-                            rays_per_launch, 16)
+                            workgroups_per_cu, 16, idle_exit_us)
         h = C.c_void_p()
         check(ctx.lib.meso_serve_start(ctx.h, byref(cfg), C.byref(h)))
         self.h = h

@@ -74,10 +74,10 @@ def plain_run(ctx, scenes, reqs, sh, timed_arrivals):
     return np.array(lat), time.perf_counter() - start
 
 
-def served_run(ctx, scenes, reqs, sh, budget_bytes, timed_arrivals, rays_per_launch=1 << 20,
-               max_inflight=2):
+def served_run(ctx, scenes, reqs, sh, budget_bytes, timed_arrivals, max_inflight=64,
+               workgroups_per_cu=0):
     srv = R.Server(ctx, budget_bytes, max_pending=len(reqs) + 8, max_inflight=max_inflight,
-                   rays_per_launch=rays_per_launch)
+                   workgroups_per_cu=workgroups_per_cu)
     lat = [None] * len(reqs)
     tickets = [None] * len(reqs)
     lag = [0.0] * len(reqs)
@@ -169,7 +169,24 @@ def _pct(a, q):
     return float(np.percentile(a, q) * 1e3)
 
 
-def run(ctx, n_requests=400, qps=0.0, budget_bytes=512 << 20, seed=4, log=print):
+def _queue_stats(st):
+    n = max(int(st["admitted"]), 1)
+    return dict(queue_ms_mean=round(st["queue_us_sum"] / n * 1e-3, 4),
+                queue_ms_max=round(st["queue_us_max"] * 1e-3, 3),
+                queued_over_1ms=int(st["queued_over_1ms"]),
+                admission_blocked_passes=int(st["admission_blocked_passes"]),
+                refused_busy=int(st["refused_busy"]),
+                peak_budget_mb=round(st["peak_bytes_in_use"] / 2**20, 1),
+                launches=int(st["launches"]))
+
+
+def run(ctx, n_requests=400, qps=0.0, budget_bytes=512 << 20, seed=4, log=print,
+        qps_budget_bytes=32 << 20):
+    """Capacity (every request queued at t=0, budget `budget_bytes`) of the
+    plain and served paths, then latency at a fixed Poisson QPS with a budget
+    of `qps_budget_bytes`: small enough that it binds (a 1920x1080 request
+    alone holds 27 MB), so requests wait for admission while earlier results
+    are still held; the queueing that causes is reported."""
     sys.setswitchinterval(1e-4)
     edges = W.serve_scenes(seed)
     t = time.time()
@@ -181,6 +198,7 @@ def run(ctx, n_requests=400, qps=0.0, budget_bytes=512 << 20, seed=4, log=print)
     # capacity: everything queued at t=0
     cap_reqs = W.serve_requests(edges, min(n_requests, 200), 1.0, seed)
     cap_rays = sum(count_rays(ctx, scenes, cap_reqs, sh))
+    served_run(ctx, scenes, cap_reqs[:16], sh, budget_bytes, False)  # warm the server path
     _, mk_p = plain_run(ctx, scenes, cap_reqs, sh, False)
     _, mk_s, st_s = served_run(ctx, scenes, cap_reqs, sh, budget_bytes, False)
     plain_mrays = cap_rays / mk_p / 1e6
@@ -195,19 +213,23 @@ def run(ctx, n_requests=400, qps=0.0, budget_bytes=512 << 20, seed=4, log=print)
     reqs = W.serve_requests(edges, n_requests, qps, seed)
     rays_q = sum(count_rays(ctx, scenes, reqs, sh))
     lp, _ = plain_run(ctx, scenes, reqs, sh, True)
-    ls, mk_q, st_q = served_run(ctx, scenes, reqs, sh, budget_bytes, True)
+    ls, mk_q, st_q = served_run(ctx, scenes, reqs, sh, qps_budget_bytes, True)
     out = dict(qps=round(qps, 1), requests=n_requests, scenes=len(scenes), scene_edges=edges,
-               resident_mb=round(mb, 1), budget_mb=budget_bytes >> 20,
+               resident_mb=round(mb, 1), capacity_budget_mb=budget_bytes >> 20,
                capacity_mrays_served=round(served_mrays, 1),
                capacity_mrays_plain=round(plain_mrays, 1),
                capacity_ratio=round(served_mrays / plain_mrays, 3),
+               capacity_stats=_queue_stats(st_s),
+               budget_mb=qps_budget_bytes >> 20,
                p50_ms_served=round(_pct(ls, 50), 3), p99_ms_served=round(_pct(ls, 99), 3),
                p50_ms_plain=round(_pct(lp, 50), 3), p99_ms_plain=round(_pct(lp, 99), 3),
-               mrays_at_qps=round(rays_q / mk_q / 1e6, 1),
-               launches=int(st_q["launches"]),
-               peak_budget_mb=round(st_q["peak_bytes_in_use"] / 2**20, 1))
-    log("serving at %.0f QPS: served p50 %.2f p99 %.2f ms; plain p50 %.2f p99 %.2f ms" % (
-        qps, out["p50_ms_served"], out["p99_ms_served"], out["p50_ms_plain"], out["p99_ms_plain"]))
+               mrays_at_qps=round(rays_q / mk_q / 1e6, 1))
+    out.update(_queue_stats(st_q))
+    log("serving at %.0f QPS, budget %d MB: served p50 %.2f p99 %.2f ms (admission wait mean %.3f "
+        "max %.3f ms, %d over 1 ms, peak %.1f MB held); plain p50 %.2f p99 %.2f ms" % (
+            qps, qps_budget_bytes >> 20, out["p50_ms_served"], out["p99_ms_served"],
+            out["queue_ms_mean"], out["queue_ms_max"], out["queued_over_1ms"],
+            out["peak_budget_mb"], out["p50_ms_plain"], out["p99_ms_plain"]))
     for s in scenes:
         s.destroy()
     return out

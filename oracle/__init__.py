@@ -116,6 +116,9 @@ def lib():
             "mo_ref_erode_mips": (None, [P, C.c_int32, C.c_int32, P]),
             "mo_ref_generate_sphere": (C.c_int32, [C.c_int32] * 3 + [C.c_float, C.c_int32, P]),
             "mo_ref_test_generator": (C.c_int32, [C.c_int32] * 3 + [C.c_float, C.c_int32, P]),
+            "mo_ref_hash3": (C.c_double, [C.c_double] * 3),
+            "mo_ref_noised": (None, [P, P]),
+            "mo_ref_displacement": (C.c_double, [P]),
             "mo_ref_octant_id": (C.c_int32, [C.c_float] * 3),
             "mo_build_coarse": (None, [P, C.c_int64, P, P, P]),
             "mo_edit_chunks": (None, [P, P, P, P, C.c_int32, C.c_uint32]),
@@ -384,6 +387,22 @@ def ref_test_generator(cx, cy, cz, block_size=1.0, res=16):
     cells = np.zeros((res ** 3, 3), np.uint8)
     k = lib().mo_ref_test_generator(cx, cy, cz, block_size, res, _p(cells))
     return cells[:k]
+
+
+def ref_hash3(x, y, z):
+    return lib().mo_ref_hash3(float(x), float(y), float(z))
+
+
+def ref_noised(x):
+    """(dx, dy, dz, value) of the reference's gradient value noise"""
+    xi = np.ascontiguousarray(x, np.float64)
+    out = np.zeros(4, np.float64)
+    lib().mo_ref_noised(_p(xi), _p(out))
+    return out
+
+
+def ref_displacement(p):
+    return lib().mo_ref_displacement(_p(np.ascontiguousarray(p, np.float64)))
 
 
 def two_level_from_bricks(n, slots, masks, voxels):

@@ -1134,6 +1134,10 @@ static double ref_displacement(const double p0[3]) {
   return d;
 }
 
+double mo_ref_hash3(double x, double y, double z) { return ref_hash3(x, y, z); }
+void mo_ref_noised(const double x[3], double out[4]) { ref_noised(x, out); }
+double mo_ref_displacement(const double p[3]) { return ref_displacement(p); }
+
 int32_t mo_ref_test_generator(int32_t cx, int32_t cy, int32_t cz, float block_size, int32_t res,
                               uint8_t *cells) {
   int32_t k = 0;
@@ -1269,14 +1273,28 @@ void mo_pool_forward(const float raw[3], float f[3]) {
   for (int i = 0; i < 3; i++) f[i] = raw[i] / len;
 }
 
+/* GetFibonacciSphere<float> (Runtimes/Helper/VoxelMathHelper.h:47-68), as
+ * BakeVisibilityByView instantiates it (ChunkManagerHelper.h:205): Phi is
+ * computed in double and stored as float; everything after is float
+ * arithmetic (cosf / sinf / sqrtf), and glm::normalize multiplies by
+ * 1 / sqrt(dot(v, v)) with dot summed x, y, then z. */
 void mo_pool_bake_directions(int32_t n, float *dirs) {
-  const double phi = 3.14159265358979323846 * (sqrt(5.0) - 1.0);
+  const float phi = (float)(3.14159265358979323846 * (sqrt(5.0) - 1.0));
   for (int32_t i = 0; i < n; i++) {
-    const double y = 1.0 - ((double)i / (double)(n - 1)) * 2.0;
-    const double r = sqrt(1.0 - y * y);
-    const double th = phi * (double)i;
-    const float v[3] = {(float)(cos(th) * r), (float)y, (float)(sin(th) * r)};
-    mo_pool_forward(v, dirs + 3 * i);
+    float y = (float)i / (float)(n - 1);
+    y = y * 2.0f;
+    y = 1.0f - y;
+    float yy = y * y;
+    const float r = sqrtf(1.0f - yy);
+    const float th = phi * (float)i;
+    const float x = cosf(th) * r;
+    const float z = sinf(th) * r;
+    float dd = x * x + yy;
+    dd = dd + z * z;
+    const float inv = 1.0f / sqrtf(dd);
+    dirs[3 * i + 0] = x * inv;
+    dirs[3 * i + 1] = y * inv;
+    dirs[3 * i + 2] = z * inv;
   }
 }
 

@@ -248,10 +248,20 @@ void mo_ref_erode_mips(const uint8_t *occ, int32_t res, int32_t depth, uint8_t *
 int32_t mo_ref_generate_sphere(int32_t cx, int32_t cy, int32_t cz, float block_size,
                                int32_t res, uint8_t *cells);
 /* FGeneratorHelper::TestGenerator (GeneratorHelper.h:90-119) with noised /
- * displacement (:18-87) and FVoxelMathHelper::Hash (VoxelMathHelper.h:29-33).
- * Parity unpinned: depends on libm sin at large arguments. */
+ * displacement (:18-87) and FVoxelMathHelper::Hash (VoxelMathHelper.h:29-33),
+ * in double as the reference instantiates them (dvec3).  The hash's bits
+ * follow the platform libm sin at large arguments; the structure (lattice
+ * values, derivatives, octave composition, the height test) is pinned by
+ * tests/test_oracle_refnoise.py. */
 int32_t mo_ref_test_generator(int32_t cx, int32_t cy, int32_t cz, float block_size,
                               int32_t res, uint8_t *cells);
+/* FVoxelMathHelper::Hash of a dvec3 (VoxelMathHelper.h:29-33) */
+double mo_ref_hash3(double x, double y, double z);
+/* FGeneratorHelper::noised (GeneratorHelper.h:18-53); out = (dx, dy, dz, value),
+ * the .yzwx order the reference returns */
+void mo_ref_noised(const double x[3], double out[4]);
+/* FGeneratorHelper::displacement (GeneratorHelper.h:55-87) */
+double mo_ref_displacement(const double p[3]);
 /* GetOctantId (Samples/SimpleVoxel.cpp:129-136) */
 int32_t mo_ref_octant_id(float x, float y, float z);
 
@@ -292,10 +302,12 @@ typedef struct {
 } mo_pool_result;
 float mo_pool_view_threshold(float view_angle_deg);
 void mo_pool_forward(const float raw[3], float f[3]);
-/* FVoxelMathHelper::GetFibonacciSphere (Runtimes/Helper/VoxelMathHelper.h:48-71):
- * n >= 2 unit directions, y = 1 - 2i/(n-1), radius sqrt(1 - y^2), angle
- * i * pi * (sqrt(5) - 1), evaluated in double, rounded to float and
- * normalised in float (length = sqrt((x*x + y*y) + z*z), v / length). */
+/* FVoxelMathHelper::GetFibonacciSphere<float> (Runtimes/Helper/VoxelMathHelper.h:47-68),
+ * as BakeVisibilityByView instantiates it (ChunkManagerHelper.h:205): n >= 2
+ * unit directions, y = 1 - 2i/(n-1), radius sqrt(1 - y^2), angle
+ * i * pi * (sqrt(5) - 1); Phi in double stored as float, the rest float
+ * arithmetic (cosf / sinf / sqrtf), normalised as glm::normalize does:
+ * v * (1 / sqrt((x*x + y*y) + z*z)). */
 void mo_pool_bake_directions(int32_t n, float *dirs);
 /* The baked direction nearest to the unit forward f (TNearestMap::Query,
  * Runtimes/Voxel/Spatial/NearestMap.h:36-50: Euclidean nearest; the first

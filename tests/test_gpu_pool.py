@@ -155,7 +155,9 @@ def test_pool_argument_checks(ctx):
 @pytest.mark.parametrize("capacity,radius,max_loads", [
     (20000, 4, 64),     # more slots than the selection holds in registers
     (5000, 9, 4096),    # the selection's largest limit: 4,096 places sorted in LDS
-    (5000, 9, 5000)])   # a limit above the selection's, so the lists are sorted
+    (5000, 9, 5000),    # a limit above the selection's, so the lists are sorted
+    (140000, 4, 64),    # more slots than the selection takes: the lists are sorted
+    (5000, 26, 64)])    # a cube of 53^3 offsets, longer than the selection takes
 def test_pool_selection_paths(ctx, capacity, radius, max_loads):
     n, seed = 256, 5
     sc = ctx.scene_generate(L.GEN_CAVES, L.SCENE_TWO_LEVEL_U16, n, seed)
