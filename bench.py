@@ -1,17 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark of the MesoEngine voxel hot path on one MI355X.
 
-Headline (value): Mrays/s over K frames, each one 1920x1080 frame of
-primary rays plus one shadow ray per lit hit over the 512^3 heightfield brick
-map (u16 materials, 8^3 bricks), scene and outputs resident in HBM, with two
-frames in flight (consecutive frames on two streams, each into its own
-outputs).  The same frames one at a time on one stream are reported beside
-it and give the roofline's launch duration.  Also in the same JSON
-line: the meshing of 4096 chunks of 32^3 u16) batched vs plain, serving
-(continuous batching over 16 resident scenes at a fixed QPS, p50/p99), the
-2048^3 cave frame loop (1% of chunks edited and remeshed, then a 3840x2160
-frame with shadow and AO rays; alone and under serving load) and the scalar
-CPU oracle timed on this host's cores.
+Headline (value): Mrays/s over K frames of the 2048^3 two-level cave volume
+(u16, 8^3 bricks under 16^3-brick nodes), each frame: 1% of the 32^3 chunks
+edited in place, those chunks greedy-remeshed from the resident scene, and a
+3840x2160 render of primary rays plus a shadow ray per lit hit and 4
+cosine-hemisphere AO rays per hit.  Scene and outputs are resident in HBM;
+frames run back to back, each frame's remesh on a second stream beside its
+render.  The roofline is the render kernel's (two-level byte model over its
+launch duration, measured alone on the stream it runs on); the CPU baseline
+is the oracle's scalar path for the same frame on this host's cores.
+
+Also in the same JSON line: the 512^3 heightfield 1080p frame (primary +
+shadow, two frames in flight), the meshing of 4096 chunks of 32^3 u16
+batched vs plain, serving (continuous batching over 16 resident scenes:
+capacity and p50/p99 at a fixed QPS with a binding HBM budget), the frame
+loop under serving load, the brick upload path, chunk streaming and the CPU
+path's other parts.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -310,13 +315,14 @@ def mesh_leg(ctx, R, L, W, steps, warmup, use_graph=True):
     return out
 
 
-def serve_leg(ctx, R, L, W, n_requests, qps, budget_mb):
+def serve_leg(ctx, R, L, W, n_requests, qps, budget_mb, qps_budget_mb):
     from mesoengine_amd import serving
-    out = serving.run(ctx, n_requests=n_requests, qps=qps, budget_bytes=budget_mb << 20, log=log)
+    out = serving.run(ctx, n_requests=n_requests, qps=qps, budget_bytes=budget_mb << 20, log=log,
+                      qps_budget_bytes=qps_budget_mb << 20)
     # the same stream from the native load generator: capacity (200 requests
     # queued at t=0) and latency at the same QPS (1000 requests)
     cap = serving.run_native(200, 1.0, budget_bytes=budget_mb << 20, log=log, capacity=True)
-    timed = serving.run_native(1000, out["qps"], budget_bytes=budget_mb << 20, log=log)
+    timed = serving.run_native(1000, out["qps"], budget_bytes=qps_budget_mb << 20, log=log)
     out["native"] = dict(capacity_mrays_served=cap["served"]["mrays"],
                          capacity_mrays_plain=cap["plain"]["mrays"],
                          capacity_ratio=round(cap["served"]["mrays"] / cap["plain"]["mrays"], 3),
@@ -324,7 +330,71 @@ def serve_leg(ctx, R, L, W, n_requests, qps, budget_mb):
     return out
 
 
+def cpu_frame_leg(ctx, L, W, cores, seconds_cap=30.0):
+    """The CPU path of one headline frame on this host's cores: the oracle's
+    edit of the frame's dirty chunks, its greedy mesher over them, and its
+    two-level DDA (primary + shadow + 4 AO) over the whole 3840x2160 frame.
+    It runs on the GPU-generated volume downloaded to host memory, which
+    equals the oracle's generator (the GPU tests compare bricks of it)."""
+    import oracle as O
+    from mesoengine_amd import frame as F
+    cfg = W.CAVES
+    n, seed = cfg["n"], cfg["seed"]
+    t0 = time.time()
+    sc = ctx.scene_generate(L.GEN_CAVES, L.SCENE_TWO_LEVEL_U16, n, seed)
+    try:
+        a = sc.arrays()
+    finally:
+        sc.destroy()
+    download_s = time.time() - t0
+    osc = O.two_level_scene(n, a["top"], a["slots"], a["masks"], a["voxels"])
+    cp = W.caves_camera(n, cfg["width"], cfg["height"])
+    ocam = O.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], cfg["width"], cfg["height"])
+    osh = O.shading(W.LIGHT, seed, cfg["ao_samples"], cfg["ao_radius"])
+    flags = O.RAY_SHADOW | O.RAY_AO
+    # edit and remesh of one frame's dirty chunks
+    xyz = F.dirty_chunks(n, 0, seed)
+    t = time.perf_counter()
+    O.edit_chunks(osc, xyz, seed * 7919)
+    edit_s = time.perf_counter() - t
+    t = time.perf_counter()
+    chunks = np.stack([O.scene_chunk(osc, *map(int, c)) for c in xyz])
+    gather_s = time.perf_counter() - t
+    t = time.perf_counter()
+    O.mesh_chunks_counts(chunks, True, cores)
+    mesh_s = time.perf_counter() - t
+    # the render, full frame, median of up to 3 within the cap
+    times, rays = [], 0
+    t_start = time.time()
+    while len(times) < 3 and (time.time() - t_start) < seconds_cap:
+        t = time.perf_counter()
+        _, _, _, _, cnt = O.raymarch(osc, ocam, flags, osh, threads=cores)
+        times.append(time.perf_counter() - t)
+        rays = cnt.rays + cnt.shadow_rays + cnt.ao_rays
+    render_s = sorted(times)[len(times) // 2]
+    # one thread on a band of 16 rows
+    t = time.perf_counter()
+    _, _, _, _, c1 = O.raymarch(osc, ocam, flags, osh, row0=1072, row1=1088, threads=1)
+    st = time.perf_counter() - t
+    frame_s = edit_s + gather_s + mesh_s + render_s
+    return dict(value=round(rays / frame_s / 1e6, 2), unit="Mrays/s", cores=cores, kind="port",
+                sample="one whole headline frame on %d threads: the oracle's edit of the %d dirty "
+                       "chunks (%.3fs), their voxels gathered from the volume (%.3fs) and "
+                       "greedy-meshed (%.3fs), and the two-level DDA over the full 3840x2160 frame "
+                       "with shadow and 4 AO rays (median of %d: %.3fs); volume generated on the "
+                       "GPU and downloaded in %.1fs (not timed)" % (
+                           cores, len(xyz), edit_s, gather_s, mesh_s, len(times), render_s,
+                           download_s),
+                frame_ms=round(frame_s * 1e3, 1), render_ms=round(render_s * 1e3, 1),
+                render_mrays=round(rays / render_s / 1e6, 2),
+                remesh_mvox_s=round(len(xyz) * 32768 / (gather_s + mesh_s) / 1e6, 1),
+                single_thread_mrays=round((c1.rays + c1.shadow_rays + c1.ao_rays) / st / 1e6, 3),
+                rays_per_frame=int(rays))
+
+
 def cpu_leg(W, seconds_cap=30.0):
+    """The CPU path's other parts: the 1080p heightfield frame, host facts,
+    the reference's own generators, the mesher sample and the pool sample."""
     import oracle as O
     cfg = W.TERRAIN
     cores = host_cores()
@@ -347,10 +417,11 @@ def cpu_leg(W, seconds_cap=30.0):
     t = time.perf_counter()
     _, _, _, _, c1 = O.raymarch(sc, cam, O.RAY_SHADOW, sh, row0=508, row1=572, threads=1)
     st = time.perf_counter() - t
-    out = dict(value=rays / med / 1e6, unit="Mrays/s", cores=cores, kind="port",
-               sample="full 1920x1080 frame (primary + shadow) of the 512^3 heightfield brick map, "
-                      "median of %d runs on %d threads; scene build %.2fs" % (len(times), cores, gen_s),
-               single_thread_mrays=(c1.rays + c1.shadow_rays) / st / 1e6)
+    out = dict(heightfield_1080p=dict(
+        mrays=round(rays / med / 1e6, 2), cores=cores,
+        sample="full 1920x1080 frame (primary + shadow) of the 512^3 heightfield brick map, "
+               "median of %d runs on %d threads; scene build %.2fs" % (len(times), cores, gen_s),
+        single_thread_mrays=round((c1.rays + c1.shadow_rays) / st / 1e6, 3)))
     out.update(cpu_extras(O, cores))
     out["chunk_streaming"] = cpu_pool_sample(O, sc)
     return out
@@ -427,17 +498,20 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline frame loop (for profiling runs)")
     ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-frame", action="store_true",
-                    help="skip the 2048^3 cave frame loop (edit + remesh + render)")
-    ap.add_argument("--frames", type=int, default=8)
+    ap.add_argument("--frames", type=int, default=8, help="frames under serving load")
     ap.add_argument("--no-graph", action="store_true",
                     help="batched meshing launched directly instead of by graph replay")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--qps", type=float, default=0.0, help="0 = derive from plain capacity")
     ap.add_argument("--budget-mb", type=int, default=512)
+    ap.add_argument("--qps-budget-mb", type=int, default=32,
+                    help="HBM budget of the fixed-QPS serving run (binds: a 1080p request "
+                         "holds 27 MB)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -445,81 +519,101 @@ def main():
         sys.exit(2)
 
     from mesoengine_amd import _lib as L
+    from mesoengine_amd import frame as F
     from mesoengine_amd import runtime as R
     from mesoengine_amd import workloads as W
 
     ctx = R.Context(0)
     info = ctx.device_info()
     log("device %s, %d CUs" % (info.name.decode(), info.compute_units))
-    rm = raymarch_leg(ctx, R, L, W, args.steps, args.warmup)
-    log("raymarch: %.3f ms/frame, %.1f Mrays/s (%.1f primary), %.0f GB/s algorithmic; "
-        "two frames in flight %.3f ms/frame, %.1f Mrays/s" % (
-            rm["ms"], rm["mrays"], rm["mrays_primary"], rm["achieved_gbs"], rm["ms_pipe"],
-            rm["mrays_pipe"]))
-    mesh = None if args.no_mesh else mesh_leg(ctx, R, L, W, args.steps, args.warmup,
-                                                 not args.no_graph)
-    serve = None
-    if not args.no_serve:
-        serve = serve_leg(ctx, R, L, W, args.serve_requests, args.qps, args.budget_mb)
-    frame = None
-    if not args.no_frame:
-        from mesoengine_amd import frame as F
-        qps = serve["qps"] if serve else (args.qps or 1000.0)
-        frame = F.run(ctx, frames=args.frames, warmup=2, serve_qps=qps,
-                      serve_requests=args.serve_requests, budget_bytes=args.budget_mb << 20,
-                      log=log)
-    upload = None if args.no_frame else upload_leg()
-    streaming = None
-    if not args.no_frame:
+    head, fr = F.headline(ctx, args.steps, args.warmup, log=log)
+    rays = head["rays_per_frame"]
+    loaded = rm = mesh = serve = upload = streaming = None
+    try:
+        if not args.headline_only:
+            qps = 1000.0
+            if not args.no_serve:
+                serve = serve_leg(ctx, R, L, W, args.serve_requests, args.qps, args.budget_mb,
+                                  args.qps_budget_mb)
+                qps = serve["qps"]
+            loaded = F.under_serving_load(ctx, fr, rays, frames=args.frames, serve_qps=qps,
+                                          serve_requests=args.serve_requests,
+                                          budget_bytes=args.budget_mb << 20, log=log)
+    finally:
+        fr.close()
+    if not args.headline_only:
+        rm = raymarch_leg(ctx, R, L, W, args.steps, args.warmup)
+        log("1080p heightfield: %.3f ms/frame, %.1f Mrays/s; two frames in flight %.3f ms/frame, "
+            "%.1f Mrays/s" % (rm["ms"], rm["mrays"], rm["ms_pipe"], rm["mrays_pipe"]))
+        mesh = None if args.no_mesh else mesh_leg(ctx, R, L, W, args.steps, args.warmup,
+                                                     not args.no_graph)
+        upload = upload_leg()
         from mesoengine_amd import streaming as S
         streaming = S.run(ctx, log=log)
-    cpu = None if args.no_cpu else cpu_leg(W)
-    traffic = None
+    cpu = cpu_other = None
+    if not args.no_cpu:
+        cores = host_cores()
+        cpu = cpu_frame_leg(ctx, L, W, cores)
+        log("cpu baseline: %.1f Mrays/s for the whole frame on %d threads (%s)" % (
+            cpu["value"], cores, cpu["sample"]))
+        if not args.headline_only:
+            cpu_other = cpu_leg(W)
+    traffic, traffic_src = None, None
     tp = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tp):
-        # the PMC traffic of the launch shape pinned for the timed frames
+        # PMC bytes per launch of the headline's render kernel, from the
+        # rocprofv3 passes recorded in profiles/ (not measured by this run)
         shape = {5: "p8x8", 6: "p16x4", 7: "p32x2", 8: "pair8x8", 9: "pair16x4"}.get(
-            rm["tune"]["two_in_flight"]["variant"], "tiled")
+            head["render_tune"]["variant"], "tiled")
+        key = "raymarch_two_level_caves2048_" + shape
         with open(tp) as f:
-            k = json.load(f).get("kernels", {}).get("raymarch_brickmap_terrain512_" + shape)
-            traffic = k["hbm_bytes"] if k else None
+            k = json.load(f).get("kernels", {}).get(key)
+        if k:
+            traffic = k["hbm_bytes"]
+            traffic_src = "profiles/traffic.json:%s (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)" % key
+    achieved = head["render_alg_gbs"]
     line = {
         "metric": "Mrays/s and p99 latency at fixed QPS; Mvoxels meshed/s; batched vs plain",
-        "value": round(rm["mrays_pipe"], 2),
+        "value": round(head["mrays"], 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(rm["ms_pipe"], 4),
+        "ms_per_step": round(head["ms_per_frame"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+int32",
+        "dtype": "f32+int64",
         "data": "synthetic (seeded procedural scenes generated on the GPU)",
-        "config": {"workload": "512^3 heightfield brick map (8^3 bricks, u16), 1920x1080 "
-                               "primary + 1 shadow ray per lit hit", "rays_per_frame": rm["rays"],
-                   "primary_rays": rm["primary"], "shadow_rays": rm["shadow"],
-                   "frames_in_flight": 2},
-        "roofline": {"bound": "hbm", "achieved": round(rm["achieved_gbs"], 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(rm["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "alg_bytes_per_launch": rm["alg_bytes"],
-                     "layout_bytes_per_launch": rm["layout_bytes"]},
+        "config": {"workload": head["workload"], "rays_per_frame": rays,
+                   "primary_rays": head["counts"]["rays"],
+                   "shadow_rays": head["counts"]["shadow_rays"],
+                   "ao_rays": head["counts"]["ao_rays"], "dirty_chunks": head["dirty_chunks"],
+                   "scene": head["scene"]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": "render (raymarch, two-level, variant %d)" % (
+                         head["render_tune"]["variant"]),
+                     "launch_ms": round(head["render_ms"], 4),
+                     "alg_bytes_per_launch": head["render_alg_bytes"]},
         "cpu_baseline": cpu,
-        "mrays_primary": round(rm["mrays_primary_pipe"], 2),
-        # one frame at a time on one stream: the launch duration the roofline
-        # and the kernel trace use
-        "raymarch_one_stream": {"ms_per_frame": round(rm["ms"], 4),
-                                "mrays": round(rm["mrays"], 2),
-                                "wall_ms_per_frame": round(rm["wall_ms"], 4)},
-        "raymarch_tune": rm["tune"],
-        "raymarch_pcie_inclusive": dict(rm["pcie_inclusive"], mrays=round(
-            rm["rays"] / (rm["pcie_inclusive"]["ms_per_frame"] * 1e-3) / 1e6, 1)),
+        "frame": {k: head[k] for k in ("render_ms", "render_mrays", "render_tune",
+                                       "split_one_stream", "remesh_mvox_s", "remesh_alg_gbs",
+                                       "counts")},
+        "frame_under_serving_load": loaded,
+        "heightfield_1080p": None if rm is None else dict(
+            ms_pipe=round(rm["ms_pipe"], 4), mrays_pipe=round(rm["mrays_pipe"], 2),
+            ms_one_stream=round(rm["ms"], 4), mrays_one_stream=round(rm["mrays"], 2),
+            alg_bytes=rm["alg_bytes"], achieved_gbs=round(rm["achieved_gbs"], 1),
+            frac=round(rm["achieved_gbs"] / HBM_PEAK_GBS, 4), tune=rm["tune"],
+            pcie_inclusive=dict(rm["pcie_inclusive"], mrays=round(
+                rm["rays"] / (rm["pcie_inclusive"]["ms_per_frame"] * 1e-3) / 1e6, 1))),
         "meshing": mesh,
         "serving": serve,
-        "frame": frame,
         "brick_upload": upload,
         "chunk_streaming": streaming,
+        "cpu_other": cpu_other,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

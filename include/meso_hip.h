@@ -316,6 +316,16 @@ meso_status meso_mesh_chunks(meso_context *ctx, const meso_chunk_source *src, ui
                              uint64_t capacity, uint64_t *d_total, void *d_work,
                              meso_stream stream);
 
+/* Whether the batched meshing that last used d_work completed its look-back: waits for
+ * `stream`, then RUNTIME_ERROR if a chunk's bounded wait for the chunks
+ * ticketed before it gave up (its prefix, quads and the total are then
+ * wrong).  The wait is bounded so that every wave of the persistent launch
+ * exits whatever happens to the others. */
+meso_status meso_mesh_check(meso_context *ctx, const void *d_work, meso_stream stream);
+/* Polls of that wait before it gives up (default 2^24; 0 gives up at the
+ * first look, which tests use to force the failure from the host). */
+meso_status meso_mesh_set_lookback_spin_limit(meso_context *ctx, uint32_t limit);
+
 /* Batched meshing of 32^3 chunks read straight from a resident brick map or
  * two-level scene (chunk coordinates d_chunk_xyz[3k] on the device, on the
  * grid of ceil(n/32) chunks per axis; voxels past the scene edge and chunks

@@ -29,6 +29,10 @@ meso_status fail(meso_status code, const char *fmt, ...) {
   vsnprintf(buf, sizeof(buf), fmt, ap);
   va_end(ap);
   g_error = buf;
+  // the error is now this call's status: consume the runtime's per-thread
+  // last error too, or the next launch check (hipGetLastError after a
+  // kernel launch) would report a failure that was already returned
+  (void)hipGetLastError();
   return code;
 }
 

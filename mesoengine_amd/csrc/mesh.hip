@@ -27,6 +27,7 @@
 //     its output offset by an ordered decoupled look-back over the chunks
 //     ticketed before it, so one persistent launch writes one compact quad
 //     buffer plus per-chunk counts and exclusive prefixes.
+#include <cstddef>
 #include <type_traits>
 
 #include "meso_internal.h"
@@ -456,6 +457,7 @@ __device__ __forceinline__ void plane_emit(MeshLds &L, int t, uint32_t *out, uin
   for (int v = 0; v < 32; v++) {
     const uint32_t F = face_row(L, p, v);
     const uint32_t eh = L.M[p.eu + v * p.stride];
+    const uint32_t ev = L.M[p.ev + v * p.stride];
     if (v > 0) {
       const uint32_t dead = segment_fill(open & ~(F & ev_prev), open & ~starts) & open;
       close_quads(r, f, s, starts & dead, open, starts, v, out, base, cap, count);
@@ -467,7 +469,7 @@ __device__ __forceinline__ void plane_emit(MeshLds &L, int t, uint32_t *out, uin
     open |= avail;
     starts |= st;
     record_starts(r, st, v);
-    ev_prev = L.M[p.ev + v * p.stride];
+    ev_prev = ev;
   }
   close_quads(r, f, s, starts, open, starts, 32, out, base, cap, count);
 }
@@ -593,7 +595,7 @@ struct SceneSource {
 template <int ES, bool GREEDY, int SRC>
 __global__ void __launch_bounds__(kMeshBlock) mesh_batched_kernel(
     meso_chunk_source src, SceneSource ss, uint32_t *counts, uint32_t *prefix, uint32_t *quads,
-    uint64_t capacity, unsigned long long *total, MeshWork *work) {
+    uint64_t capacity, unsigned long long *total, MeshWork *work, uint32_t spin_limit) {
   __shared__ MeshLds L;
   const int t = (int)threadIdx.x;
   const uint32_t n_chunks = (uint32_t)(SRC == 0 ? src.n_chunks : ss.n_chunks);
@@ -628,6 +630,15 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_batched_kernel(
       int64_t j = (int64_t)c - 1;
       uint32_t spins = 0;
       while (j >= 0) {
+        // chunk j was ticketed earlier, so its workgroup is resident and will
+        // publish; the bound (the context's spin limit, 2^24 polls unless the
+        // host lowers it to inject the failure) only guarantees that every
+        // wave exits.  The chunk's outputs are then wrong and the error word
+        // says so (meso_mesh_check).
+        if (spins >= spin_limit) {
+          atomicOr(&work->error, 1u);
+          break;
+        }
         const uint64_t st = __hip_atomic_load(&work->status[j], __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
         if (st & kInc) {
@@ -639,12 +650,7 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_batched_kernel(
           j--;
           continue;
         }
-        // chunk j was ticketed earlier, so its workgroup is resident and will
-        // publish; the bound only guarantees that every wave exits
-        if (++spins > (1u << 24)) {
-          atomicOr(&work->error, 1u);
-          break;
-        }
+        spins++;
         __builtin_amdgcn_s_sleep(1);
       }
       __hip_atomic_store(&work->status[c], kInc | (before + chunk_total), __ATOMIC_RELAXED,
@@ -762,7 +768,8 @@ static meso_status launch_batched_t(meso_context *ctx, const meso_chunk_source &
   if (st != MESO_OK) return st;
   hipLaunchKernelGGL((mesh_batched_kernel<ES, GREEDY, SRC>), dim3(grid), dim3(kMeshBlock), 0,
                      stream, src, ss, d_counts, d_prefix, d_quads, capacity,
-                     (unsigned long long *)d_total, (MeshWork *)d_work);
+                     (unsigned long long *)d_total, (MeshWork *)d_work,
+                     ctx->lookback_spin_limit);
   MESO_HIP(hipGetLastError());
   return MESO_OK;
 }
@@ -897,6 +904,30 @@ uint64_t meso_mesh_workspace_bytes(int32_t n_chunks) {
   if (n_chunks < 1) n_chunks = 1;
   return 256u + 8u * (uint64_t)n_chunks;
 }
+
+meso_status meso_mesh_set_lookback_spin_limit(meso_context *ctx, uint32_t limit) {
+  MESO_CHECK(ctx, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL context");
+  ctx->lookback_spin_limit = limit;
+  return MESO_OK;
+}
+
+meso_status meso_mesh_check(meso_context *ctx, const void *d_work, meso_stream stream) {
+  MESO_CHECK(ctx, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL context");
+  const void *w = d_work;
+  MESO_CHECK(w, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "no workspace");
+  MESO_HIP(hipSetDevice(ctx->device));
+  const hipStream_t s = stream_of(ctx, stream);
+  MESO_HIP(hipStreamSynchronize(s));
+  unsigned int err = 0;
+  MESO_HIP(hipMemcpy(&err, (const uint8_t *)w + offsetof(MeshWork, error), sizeof(err),
+                     hipMemcpyDeviceToHost));
+  MESO_CHECK(err == 0, MESO_ERR_RUNTIME,
+             "batched meshing: a chunk's look-back wait for its predecessors gave up after %u "
+             "polls; its prefix, quads and the total are wrong",
+             ctx->lookback_spin_limit);
+  return MESO_OK;
+}
+
 
 meso_status meso_mesh_chunk(meso_context *ctx, const void *d_voxels, int32_t elem_bytes,
                             uint32_t mode, uint32_t *d_count, uint32_t *d_quads, uint32_t capacity,

@@ -109,6 +109,8 @@ struct meso_context {
   // launches: the last wave of each launch resets them)
   std::mutex queue_mu;
   std::map<hipStream_t, unsigned int *> band_queues;
+  // polls of the batched mesher's look-back wait before it gives up
+  uint32_t lookback_spin_limit = 1u << 24;
 };
 
 struct meso_buffer {

@@ -7,11 +7,13 @@ Each frame:
      scene in one persistent launch (meso_mesh_scene_chunks);
   3. render: one 3840x2160 frame of primary rays + a shadow ray per lit hit +
      4 cosine-hemisphere AO rays per hit (meso_raymarch with SHADOW | AO).
-The three steps run back to back on the context stream and are timed with
-events between them.  run() times the frame alone, then again while a
-serving session (the 16-scene request stream of the serving workload) runs
-on the server's own streams, and reports the frame split and the served
-latency under that load.
+headline() times K frames back to back with each frame's remesh on a second
+stream beside its render (the remesh and the render only read the edited
+scene; the next frame's edit waits for both), the render alone (the
+roofline's launch duration), and the three steps one after another on one
+stream (the split).  under_serving_load() runs the frames while a serving
+session (the 16-scene request stream of the serving workload) runs on the
+server's stream, and reports the frame split and the served latency.
 """
 import threading
 import time
@@ -121,6 +123,43 @@ class _Frames:
                         total))
         return out
 
+    def run_overlapped(self, frames, stream2, first=0):
+        """`frames` frames back to back, timed as one interval.  Each frame's
+        edit runs on the context stream; its remesh runs on `stream2` beside
+        its render (both only read the edited scene); the next frame's edit
+        waits for both.  Returns ms per frame."""
+        lists = [self.ctx.upload(dirty_chunks(self.cfg["n"], first + f, self.seed).ravel())
+                 for f in range(frames)]
+        self.ctx.sync()
+        L.check(self.ctx.lib.meso_stream_sync(self.ctx.h, stream2))
+        a, b, ev_edit, ev_mesh = (R.Event(self.ctx) for _ in range(4))
+        a.record()
+        for f in range(frames):
+            R.edit_chunks(self.scene, lists[f], self.n_dirty, self.seed * 7919 + first + f)
+            ev_edit.record()
+            L.check(self.ctx.lib.meso_stream_wait_event(self.ctx.h, stream2, ev_edit.h))
+            R.mesh_scene_chunks(self.ctx, self.scene, lists[f], self.n_dirty, L.MESH_GREEDY,
+                                self.mt, stream2)
+            ev_mesh.record(stream2)
+            R.raymarch(self.ctx, self.scene, self.cam, self.flags, self.sh, self.tg)
+            L.check(self.ctx.lib.meso_stream_wait_event(self.ctx.h, None, ev_mesh.h))
+        b.record()
+        b.sync()
+        return a.elapsed_ms(b) / frames
+
+    def render_alone(self, k):
+        """k renders back to back on the context stream (the stream the
+        kernel runs on): the render kernel's average launch duration"""
+        for _ in range(2):
+            R.raymarch(self.ctx, self.scene, self.cam, self.flags, self.sh, self.tg)
+        e0, e1 = R.Event(self.ctx), R.Event(self.ctx)
+        e0.record()
+        for _ in range(k):
+            R.raymarch(self.ctx, self.scene, self.cam, self.flags, self.sh, self.tg)
+        e1.record()
+        e1.sync()
+        return e0.elapsed_ms(e1) / k
+
     def close(self):
         self.scene.destroy()
 
@@ -143,8 +182,10 @@ def _under_load(ctx, fr, scenes, reqs, budget_bytes, frames, rays, log, label):
 
     def serve():
         try:
+            # the server's persistent kernel on a quarter of the wave slots
+            # (2 workgroups of 4 waves per CU), so the render keeps the rest
             result["lat"], result["makespan"], result["stats"] = serving.served_run(
-                ctx, scenes, reqs, sh, budget_bytes, True)
+                ctx, scenes, reqs, sh, budget_bytes, True, workgroups_per_cu=2)
         except Exception as ex:  # surfaced after join
             result["error"] = ex
 
@@ -174,8 +215,11 @@ def _under_load(ctx, fr, scenes, reqs, budget_bytes, frames, rays, log, label):
     return out
 
 
-def run(ctx, frames=8, warmup=2, serve_qps=1000.0, serve_requests=400, budget_bytes=512 << 20,
-        seed=5, log=print):
+def headline(ctx, steps=20, warmup=3, seed=5, log=print):
+    """The 2048^3 caves frame loop as the benchmark's headline: K frames back
+    to back (edit, then remesh beside the render), the render alone for the
+    roofline, and the frame split.  Returns (result, frames object); the
+    caller closes the frames object."""
     cfg = W.CAVES
     fr = _Frames(ctx, cfg, seed, log)
     try:
@@ -184,42 +228,59 @@ def run(ctx, frames=8, warmup=2, serve_qps=1000.0, serve_requests=400, budget_by
         cnt = fr.count()
         rays = cnt["rays"] + cnt["shadow_rays"] + cnt["ao_rays"]
         alg = frame_bytes(cnt)
-        fr.run(warmup, first=10_000)
-        alone = _summary(fr.run(frames))
-        alone["render_mrays"] = round(rays / (alone["render_ms"] * 1e-3) / 1e6, 1)
-        alone["render_alg_gbs"] = round(alg / (alone["render_ms"] * 1e-3) / 1e9, 1)
-        remesh_bytes = 65536 * fr.n_dirty + 4 * alone["quads_per_frame"] + 8 * fr.n_dirty
-        alone["remesh_mvox_s"] = round(fr.n_dirty * 32768 / (alone["remesh_ms"] * 1e-3) / 1e6, 1)
-        alone["remesh_alg_gbs"] = round(remesh_bytes / (alone["remesh_ms"] * 1e-3) / 1e9, 1)
-        log("frame alone: edit %.3f ms, remesh %.3f ms, render %.3f ms (%.0f Mrays/s), "
-            "frame %.3f ms" % (alone["edit_ms"], alone["remesh_ms"], alone["render_ms"],
-                               alone["render_mrays"], alone["frame_ms"]))
-
-        # the same frames while the serving stream runs on the server's streams,
-        # with three render shapes: the one tuned for the frame alone
-        # (persistent waves on every slot), 16x16 tiles (slots return to the
-        # dispatcher as tiles finish), persistent waves on half of the slots
-        edges = W.serve_scenes(4)
-        scenes = serving.make_scenes(ctx, edges, 4)
-        reqs = W.serve_requests(edges, serve_requests, serve_qps, 4)
-        loaded = {}
+        import ctypes as C
+        s2 = C.c_void_p()
+        L.check(ctx.lib.meso_stream_create(ctx.h, C.byref(s2)))
         try:
-            for label, pin in (("tuned", None), ("tiled_16x16", (0, 0)),
-                               ("persistent_16_of_32_slots", (5, 16))):
-                if pin:
-                    fr.pin(*pin)
-                loaded[label] = _under_load(ctx, fr, scenes, reqs, budget_bytes, frames, rays,
-                                            log, label)
-                loaded[label]["serving"]["qps"] = serve_qps
-                loaded[label]["serving"]["requests"] = serve_requests
+            fr.run_overlapped(warmup, s2, first=10_000)
+            ms = fr.run_overlapped(steps, s2, first=0)
         finally:
-            for s in scenes:
-                s.destroy()
-        return dict(workload="2048^3 two-level caves (u16, 8^3 bricks under 16^3-brick nodes), "
-                             "3840x2160 primary + shadow + 4 AO rays per hit; 1% of 32^3 chunks "
-                             "edited and greedy-remeshed per frame",
-                    scene=fr.info, dirty_chunks=fr.n_dirty, counts=cnt, rays_per_frame=rays,
-                    render_tune=tune,
-                    render_alg_bytes=alg, alone=alone, under_serving_load=loaded)
-    finally:
+            L.check(ctx.lib.meso_stream_destroy(ctx.h, s2))
+        render_ms = fr.render_alone(steps)
+        split = _summary(fr.run(min(steps, 8), first=1_000))
+        remesh_bytes = 65536 * fr.n_dirty + 4 * split["quads_per_frame"] + 8 * fr.n_dirty
+        out = dict(
+            workload="2048^3 two-level caves (u16, 8^3 bricks under 16^3-brick nodes), 3840x2160 "
+                     "primary + shadow + 4 AO rays per hit; 1% of 32^3 chunks edited and "
+                     "greedy-remeshed per frame",
+            scene=fr.info, dirty_chunks=fr.n_dirty, counts=cnt, rays_per_frame=rays,
+            render_tune=tune, ms_per_frame=ms, mrays=rays / (ms * 1e-3) / 1e6,
+            render_ms=render_ms, render_mrays=rays / (render_ms * 1e-3) / 1e6,
+            render_alg_bytes=alg, render_alg_gbs=alg / (render_ms * 1e-3) / 1e9,
+            split_one_stream=split,
+            remesh_mvox_s=round(fr.n_dirty * 32768 / (split["remesh_ms"] * 1e-3) / 1e6, 1),
+            remesh_alg_gbs=round(remesh_bytes / (split["remesh_ms"] * 1e-3) / 1e9, 1))
+        log("frame loop: %.3f ms per frame (edit, then remesh beside the render), %.0f Mrays/s; "
+            "render alone %.3f ms (%.0f GB/s algorithmic); one stream: edit %.3f, remesh %.3f, "
+            "render %.3f ms" % (ms, out["mrays"], render_ms, out["render_alg_gbs"],
+                                split["edit_ms"], split["remesh_ms"], split["render_ms"]))
+        return out, fr
+    except Exception:
         fr.close()
+        raise
+
+
+def under_serving_load(ctx, fr, rays, frames=8, serve_qps=1000.0, serve_requests=400,
+                       budget_bytes=512 << 20, log=print):
+    """The frame loop of `fr` while the serving stream (the 16-scene request
+    stream of the serving workload) runs on the server's own stream, with
+    three render shapes: the one tuned for the frame alone (persistent waves
+    on every slot), 16x16 tiles (slots return to the dispatcher as tiles
+    finish), persistent waves on half of the slots."""
+    edges = W.serve_scenes(4)
+    scenes = serving.make_scenes(ctx, edges, 4)
+    reqs = W.serve_requests(edges, serve_requests, serve_qps, 4)
+    loaded = {}
+    try:
+        for label, pin in (("tuned", None), ("tiled_16x16", (0, 0)),
+                           ("persistent_16_of_32_slots", (5, 16))):
+            if pin:
+                fr.pin(*pin)
+            loaded[label] = _under_load(ctx, fr, scenes, reqs, budget_bytes, frames, rays,
+                                        log, label)
+            loaded[label]["serving"]["qps"] = serve_qps
+            loaded[label]["serving"]["requests"] = serve_requests
+    finally:
+        for s in scenes:
+            s.destroy()
+    return loaded

@@ -122,9 +122,49 @@ class FakeDevice final : public meso_serve::ServeDevice<Payload> {
   std::thread worker_;
 };
 
+// A device that stops working: after `fail_after` publications, tick()
+// reports an error (as a failed kernel launch would).
+class FailingDevice final : public meso_serve::ServeDevice<Payload> {
+ public:
+  explicit FailingDevice(int fail_after) : fail_after_(fail_after) {}
+  void publish(uint64_t, uint32_t, uint64_t, uint32_t, const Payload &) override { published_++; }
+  bool completed(uint32_t, uint64_t) override { return false; }  // never completes
+  bool tick(bool) override { return published_ < fail_after_; }
+  std::string error() override { return "injected device failure"; }
+  uint64_t launches() override { return 0; }
+  double busy_ms() override { return 0.0; }
+  void shutdown() override {}
+
+ private:
+  int fail_after_;
+  int published_ = 0;  // scheduler thread only
+};
+
+void device_failure_phase() {
+  FailingDevice dev(3);
+  meso_serve::CoreConfig cfg;
+  cfg.budget_bytes = 1 << 20;
+  cfg.max_pending = 64;
+  cfg.ring_entries = 4;
+  auto core = std::make_unique<meso_serve::ServeCore<Payload>>(cfg, &dev);
+  core->start();
+  std::vector<uint64_t> tickets(10);
+  Payload p;
+  for (auto &t : tickets) EXPECT(core->submit(p, 1024, 16, 16, &t) == MESO_OK);
+  // every request fails: the three published and the ones still queued
+  for (uint64_t t : tickets) EXPECT(core->wait(t, 5000000, nullptr) == MESO_ERR_RUNTIME);
+  int32_t done = 0;
+  EXPECT(core->poll(tickets[0], &done) && done == -1);
+  uint64_t t2 = 0;
+  EXPECT(core->submit(p, 1024, 16, 16, &t2) == MESO_ERR_RUNTIME);
+  EXPECT(core->error() == "injected device failure");
+  core->stop();
+}
+
 }  // namespace
 
 int main() {
+  device_failure_phase();
   const uint64_t budget = 1 << 20;
   const uint32_t ring = 8;
   FakeDevice dev(ring, budget);

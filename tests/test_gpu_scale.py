@@ -225,7 +225,7 @@ def test_served_requests_on_256_to_1024_scenes_match_oracle(ctx):
             ref = O.raymarch(O.brickmap_scene(e, a["slots"], a["masks"], a["voxels"]), ocam,
                              O.RAY_SHADOW, O.shading(W.LIGHT), threads=16)
             _equal(res[i], ref, "served request on the %d^3 scene" % e)
-            assert (res[i][0] >= 0).mean() > 0.05
+            assert (res[i][0] >= 0).sum() > 5000  # not a frame of sky
     finally:
         for s in scenes:
             s.destroy()
