@@ -316,7 +316,8 @@ meso_status meso_mesh_chunks(meso_context *ctx, const meso_chunk_source *src, ui
                              uint64_t capacity, uint64_t *d_total, void *d_work,
                              meso_stream stream);
 
-/* Whether the batched meshing that last used d_work completed its look-back: waits for
+/* Whether the batched meshing that last used d_work (NULL: the static
+ * scratch of meso_mesh_chunks_noalloc) completed its look-back: waits for
  * `stream`, then RUNTIME_ERROR if a chunk's bounded wait for the chunks
  * ticketed before it gave up (its prefix, quads and the total are then
  * wrong).  The wait is bounded so that every wave of the persistent launch
@@ -325,6 +326,21 @@ meso_status meso_mesh_check(meso_context *ctx, const void *d_work, meso_stream s
 /* Polls of that wait before it gives up (default 2^24; 0 gives up at the
  * first look, which tests use to force the failure from the host). */
 meso_status meso_mesh_set_lookback_spin_limit(meso_context *ctx, uint32_t limit);
+
+/* meso_mesh_chunks without a workspace argument and without any allocation:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The workspace is a static device array of the
+ * library, one per GPU, sized for MESO_MESH_NOALLOC_MAX_CHUNKS chunks;
+ * larger batches return ARGUMENT_OUT_OF_RANGE.  Every call on one GPU shares
+ * that array (each launch clears it first): calls that overlap on one GPU
+ * (different streams, or threads) race; calls ordered on one stream do not.
+ * Outputs and results are those of meso_mesh_chunks. */
+#define MESO_MESH_NOALLOC_MAX_CHUNKS 16384
+meso_status meso_mesh_chunks_noalloc(meso_context *ctx, const meso_chunk_source *src,
+                                     uint32_t mode, uint32_t *d_counts, uint32_t *d_prefix,
+                                     uint32_t *d_quads, uint64_t capacity, uint64_t *d_total,
+                                     meso_stream stream);
 
 /* Batched meshing of 32^3 chunks read straight from a resident brick map or
  * two-level scene (chunk coordinates d_chunk_xyz[3k] on the device, on the

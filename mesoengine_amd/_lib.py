@@ -166,6 +166,7 @@ _SIGS = {
     "meso_mesh_chunk": (C.c_int32, [P, P, C.c_int32, C.c_uint32, P, P, C.c_uint32, P]),
     "meso_mesh_workspace_bytes": (C.c_uint64, [C.c_int32]),
     "meso_mesh_chunks": (C.c_int32, [P, P, C.c_uint32, P, P, P, C.c_uint64, P, P, P]),
+    "meso_mesh_chunks_noalloc": (C.c_int32, [P, P, C.c_uint32, P, P, P, C.c_uint64, P, P]),
     "meso_mesh_check": (C.c_int32, [P, P, P]),
     "meso_mesh_set_lookback_spin_limit": (C.c_int32, [P, C.c_uint32]),
     "meso_mesh_graph_create": (C.c_int32, [P, P, C.c_uint32, P, P, P, C.c_uint64, P, P, PP]),

@@ -448,6 +448,12 @@ meso_status meso_context_create(int32_t device, meso_context **out) {
     delete ctx;
     return fail(MESO_ERR_RUNTIME, "context setup failed: %s", hipGetErrorString(e));
   }
+  const meso_status st = init_mesh_scratch(ctx);
+  if (st != MESO_OK) {
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return st;
+  }
   *out = ctx;
   return MESO_OK;
 }

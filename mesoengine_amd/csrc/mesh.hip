@@ -774,6 +774,24 @@ static meso_status launch_batched_t(meso_context *ctx, const meso_chunk_source &
   return MESO_OK;
 }
 
+// Scratch of meso_mesh_chunks_noalloc: the mesh workspace (ticket, error word
+// and one look-back status per chunk) for up to MESO_MESH_NOALLOC_MAX_CHUNKS
+// chunks, 131,328 bytes, held on each device the library runs on until the
+// process ends.  Host code never passes this name to a kernel: it stands for
+// a host-side shadow; init_mesh_scratch takes each device's own address.
+constexpr uint64_t kMeshScratchBytes = 256u + 8u * (uint64_t)MESO_MESH_NOALLOC_MAX_CHUNKS;
+static __device__ __align__(256) unsigned char scratch[kMeshScratchBytes];
+
+meso_status init_mesh_scratch(meso_context *ctx) {
+  // called by meso_context_create with ctx->device current
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(MESO_ERR_RUNTIME, "mesher scratch address: %s", hipGetErrorString(e));
+  ctx->mesh_scratch = p;
+  return MESO_OK;
+}
+
 MeshLaunch default_mesh_launch(meso_context *) { return MeshLaunch{kMeshBlock, 0}; }
 
 static meso_status check_source(const meso_chunk_source *src) {
@@ -913,7 +931,7 @@ meso_status meso_mesh_set_lookback_spin_limit(meso_context *ctx, uint32_t limit)
 
 meso_status meso_mesh_check(meso_context *ctx, const void *d_work, meso_stream stream) {
   MESO_CHECK(ctx, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL context");
-  const void *w = d_work;
+  const void *w = d_work ? d_work : ctx->mesh_scratch;
   MESO_CHECK(w, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "no workspace");
   MESO_HIP(hipSetDevice(ctx->device));
   const hipStream_t s = stream_of(ctx, stream);
@@ -928,6 +946,23 @@ meso_status meso_mesh_check(meso_context *ctx, const void *d_work, meso_stream s
   return MESO_OK;
 }
 
+meso_status meso_mesh_chunks_noalloc(meso_context *ctx, const meso_chunk_source *src, uint32_t mode,
+                                     uint32_t *d_counts, uint32_t *d_prefix, uint32_t *d_quads,
+                                     uint64_t capacity, uint64_t *d_total, meso_stream stream) {
+  MESO_CHECK(ctx, MESO_ERR_ARGUMENT_OUT_OF_RANGE, "NULL context");
+  MESO_CHECK(src && src->size >= sizeof(meso_chunk_source), MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "meso_chunk_source missing or too small");
+  MESO_CHECK(src->n_chunks >= 1 && src->n_chunks <= MESO_MESH_NOALLOC_MAX_CHUNKS,
+             MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "n_chunks %d not in 1..%d: the static scratch holds the look-back state of at most "
+             "%d chunks (meso_mesh_chunks takes any count with a workspace)",
+             src->n_chunks, MESO_MESH_NOALLOC_MAX_CHUNKS, MESO_MESH_NOALLOC_MAX_CHUNKS);
+  MESO_CHECK(ctx->mesh_scratch, MESO_ERR_RUNTIME, "mesher scratch address missing");
+  // the scratch takes the place of the workspace; launch_mesh_batched clears
+  // it with a fill kernel before the persistent launch
+  return meso_mesh_chunks(ctx, src, mode, d_counts, d_prefix, d_quads, capacity, d_total,
+                          ctx->mesh_scratch, stream);
+}
 
 meso_status meso_mesh_chunk(meso_context *ctx, const void *d_voxels, int32_t elem_bytes,
                             uint32_t mode, uint32_t *d_count, uint32_t *d_quads, uint32_t capacity,

@@ -109,6 +109,8 @@ struct meso_context {
   // launches: the last wave of each launch resets them)
   std::mutex queue_mu;
   std::map<hipStream_t, unsigned int *> band_queues;
+  // this device's copy of the mesher's static scratch (meso_mesh_chunks_noalloc)
+  void *mesh_scratch = nullptr;
   // polls of the batched mesher's look-back wait before it gives up
   uint32_t lookback_spin_limit = 1u << 24;
 };
@@ -196,6 +198,8 @@ struct PoolView {
 PoolView pool_view(const meso_chunk_pool *pool);
 hipStream_t stream_of(meso_context *ctx, meso_stream s);
 meso_status launch_zero(void *ptr, uint64_t bytes, hipStream_t stream);
+// the device address of the mesher's static scratch on ctx's (current) device
+meso_status init_mesh_scratch(meso_context *ctx);
 // the persistent raymarch's band counters for `stream` (1 KiB, zeroed once)
 meso_status band_queue(meso_context *ctx, hipStream_t stream, unsigned int **out);
 // builds the empty-space levels (occ32/occ128 or node groups) of a brick scene
