@@ -545,8 +545,8 @@ typedef struct {
   int32_t max_pending;        /* queued, not yet admitted; submit -> BUSY beyond */
   int32_t max_inflight;       /* admitted requests on the device at once: entries of the
                                  device ring the persistent kernel draws from (1..4096) */
-  int32_t workgroups_per_cu;  /* persistent workgroups of 256 lanes per CU (0: all that fit) */
-  int32_t tile;               /* pixel tile edge of a work item (0 or 16) */
+  int32_t workgroups_per_cu;  /* persistent workgroups of 4 waves per CU, 0..8 (0: all that fit) */
+  int32_t tile;               /* pixel edge of a work item: a wave's square packet (0 or 8) */
   int32_t idle_exit_us;       /* the persistent kernel returns after this long with nothing
                                  to claim, and is launched again for new work (0: 500) */
 } meso_serve_config;

@@ -5,27 +5,41 @@
 // their outputs fit an HBM arena of hbm_budget_bytes and a ring entry is
 // free; an admitted request is published as the next ring entry.
 //
-// Device side: a persistent kernel of 256-lane workgroups pulls 16x16-pixel
-// tiles (four 8x8 wave packets) of the published entries, oldest entry
-// first, so a request that arrives while others run starts on tiles freed by
-// them, inside the same launch.
-//  - `head` (device memory, 64 bits) packs the entry being drawn from, its
-//    tile count and the next tile: an atomic add claims a tile, and a claim
-//    at or past the count means the entry is exhausted.  Exhaustion is thus
-//    decided in device memory; a claimant reads the entry's descriptor (in
-//    host-coherent memory) only for a valid claim, when the entry cannot
-//    complete and its ring slot cannot be reused before that tile is done.
-//  - An exhausted head moves to the next published entry by compare-and-swap.
-//  - Every tile ends with a system-scope release add on its slot's counter
-//    (its outputs are written back to memory first); the add that completes
-//    the entry resets the counter and sets the slot's completion flag in host
-//    memory, which the scheduler thread polls.
-//  - Eight leader workgroups poll the host's published count and stop
-//    request and keep them in device memory for the others, so the claim
-//    loop of every other workgroup touches device memory only.  With nothing
-//    to claim for idle_exit_us the leaders mark the launch as ending and
-//    every workgroup returns; the scheduler launches the kernel again when
-//    work is outstanding.
+// Device side.  An entry is cut into 8x8-pixel packets (one wave pass each,
+// row-major).  Device-scope atomics and loads are performed at the memory
+// side, a few microseconds each under load, so the kernel schedules in two
+// levels and touches device-wide words about once per kChunk packets:
+//  - Workgroups of four waves claim chunks of up to kChunk consecutive
+//    packets from eight heads, one per XCD.  Head x hands out the
+//    contiguous band [u x / 8, u (x + 1) / 8) of an entry's u units (its
+//    packets, or 8 when it has fewer: units past the packets hold no work),
+//    so an XCD's workgroups sweep neighbouring packets.  A head packs the
+//    entry it draws from (22 bits), its band's unit count (21 bits) and the
+//    next unit (21 bits): an atomic add of kChunk claims.  A workgroup claims
+//    from its own XCD's head with the most units left, then from the others.
+//  - Inside a workgroup the waves take packets by tickets from an LDS
+//    counter, with no barrier.  Chunks cover consecutive tickets; the wave
+//    whose ticket is the first past the newest chunk claims the next one
+//    (the others wait in LDS), so no ticket is lost.  Installed chunks live
+//    in an LDS ring of kSlots; a slot is reused once its chunk is finished.
+//  - The wave that finishes a chunk's last packet publishes it: the waves'
+//    stores are ordered before it by a workgroup release/acquire through
+//    the LDS count, one system release fence writes them back, and a relaxed
+//    add moves the entry's counter (one line per ring slot).  The add that
+//    completes an entry resets the counter and sets the slot's completion
+//    flag in host memory, which the scheduler thread polls.
+//  - Every entry has at least one unit on every head and completes only when
+//    all units are finished, so every head passes an entry before it
+//    completes and its ring slot can be reused: a head, and a workgroup
+//    holding a chunk, only read descriptors of open entries.
+//  - One publisher wave (wave 0 of workgroup 0) is the only reader of host
+//    memory: it copies newly published descriptors into a device ring,
+//    announces them in a line per XCD, moves exhausted heads on to the next
+//    entry, passes on the host's stop request, and after idle_exit_us with
+//    every head exhausted and nothing new published marks the launch as
+//    ending; the scheduler launches the kernel again when work is
+//    outstanding.  Workgroups that find nothing to claim poll only their
+//    XCD's line.
 // Every pixel is computed by the same per-pixel code whatever else shares the
 // kernel, so a result is identical alone, in a mix and on the plain path.
 #include <algorithm>
@@ -38,27 +52,43 @@
 
 namespace meso {
 
-constexpr int kServeTile = 16;
-constexpr uint32_t kServeThreads = 256;
-constexpr uint32_t kServeLeaders = 8;  // one per XCD
-// head word: entry (22 bits) | tile count (21 bits) | cursor (21 bits)
+constexpr int kServePacket = 8;          // a wave's 8x8 pixels
+constexpr uint32_t kServeWaves = 4;      // waves of a workgroup
+constexpr uint32_t kServeThreads = 64 * kServeWaves;
+constexpr uint32_t kServeHeads = 8;      // one per XCD: workgroup i runs on XCD i % 8
+constexpr uint32_t kChunk = 32;          // packets a workgroup claims at once
+constexpr uint32_t kSlots = 4;           // installed chunks a workgroup holds
+constexpr uint32_t kRescanUs = 200;      // an idle workgroup looks at the heads this often
+constexpr uint32_t kCntStride = 64;      // completion counters one 256-byte line apart
+// head word: entry (22 bits) | unit count (21 bits) | cursor (21 bits)
 constexpr uint64_t kFieldMask = (1ull << 21) - 1;
 constexpr uint64_t kEntryMask = (1ull << 22) - 1;
 constexpr uint32_t kMaxRing = 4096;
+// an entry's packets, so that a band's units (and a cursor past them by a
+// claim from every workgroup) fit the 21-bit fields
+constexpr uint32_t kMaxPackets = 1u << 20;
 
-__host__ __device__ inline uint64_t head_word(uint64_t entry, uint64_t tiles) {
-  return ((entry & kEntryMask) << 42) | ((tiles & kFieldMask) << 21);
+__host__ __device__ inline uint64_t head_word(uint64_t entry, uint64_t units) {
+  return ((entry & kEntryMask) << 42) | ((units & kFieldMask) << 21);
+}
+// first unit of head x's band of an entry with `units` units
+__device__ __forceinline__ uint32_t band_start(uint32_t units, uint32_t x) {
+  return (uint32_t)(((uint64_t)units * x) / kServeHeads);
+}
+// units in head x's band: at least one, as units >= 8
+__device__ __forceinline__ uint32_t band_units(uint32_t units, uint32_t x) {
+  return band_start(units, x + 1) - band_start(units, x);
 }
 
-// One ring entry, in host-coherent memory (written by the host, read by the
-// kernel with system-scope loads).
+// One ring entry, in host-coherent memory (written by the host); the
+// publisher copies it into the device ring, which the workgroups read.
 struct alignas(8) ServeDesc {
   uint64_t seq;  // entry number (from 1)
-  uint32_t tiles_total, tiles_x;
+  uint32_t packets, packets_x;
+  uint32_t units;  // max(packets, 8)
   int32_t kind;
   uint32_t flags;
   int32_t index_bytes;
-  int32_t pad;
   void *index;
   uint8_t *face;
   float *t;
@@ -69,23 +99,30 @@ struct alignas(8) ServeDesc {
 };
 static_assert(sizeof(ServeDesc) % 8 == 0, "descriptor words are copied 8 bytes at a time");
 constexpr uint32_t kDescWords = sizeof(ServeDesc) / 8;
-static_assert(kDescWords <= kServeThreads, "one lane per descriptor word");
+static_assert(kDescWords <= 64, "one lane per descriptor word");
 
 // host-coherent control words
 struct ServeCtl {
   uint64_t published;  // last entry published (0: none)
-  uint32_t stop;       // the host asks every workgroup to return
+  uint32_t stop;       // the host asks every wave to return
   uint32_t pad;
 };
 
-// device-memory state (zeroed once when the server starts); done_cnt[ring]
-// follows it
-struct ServeState {
-  unsigned long long head;    // entry 0 is a placeholder with no tiles
-  unsigned long long mirror;  // the leaders' copy of ServeCtl::published
-  unsigned int exiting;       // launch number whose workgroups are returning
-  unsigned int pad;
+// device-memory state (zeroed once when the server starts); the completion
+// counters (kCntStride words apart) and the device descriptor ring follow.
+// Each head and each XCD's line is 256 bytes of its own.
+struct alignas(256) ServeHead {
+  unsigned long long word;  // entry 0 is a placeholder with no units
 };
+struct alignas(256) ServeXcd {
+  unsigned long long published;  // entries whose descriptors are in the device ring
+  unsigned int exiting;          // launch number whose waves are returning
+};
+struct ServeState {
+  ServeHead heads[kServeHeads];
+  ServeXcd xcd[kServeHeads];
+};
+static_assert(sizeof(ServeState) % 256 == 0, "the counters start on a line of their own");
 
 __device__ __forceinline__ uint64_t sys_load64(const uint64_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -96,6 +133,18 @@ __device__ __forceinline__ uint32_t sys_load32(const uint32_t *p) {
 __device__ __forceinline__ unsigned long long dev_load64(const unsigned long long *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ uint32_t dev_load32(const unsigned int *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// lane r's value, in every lane
+__device__ __forceinline__ uint32_t lane_at(uint32_t v, uint32_t r) {
+  return __builtin_amdgcn_readlane(v, r);
+}
+__device__ __forceinline__ uint64_t lane_at(uint64_t v, uint32_t r) {
+  return (uint64_t)lane_at((uint32_t)v, r) | ((uint64_t)lane_at((uint32_t)(v >> 32), r) << 32);
+}
+__device__ __forceinline__ uint32_t lane0(uint32_t v) { return lane_at(v, 0); }
+__device__ __forceinline__ uint64_t lane0(uint64_t v) { return lane_at(v, 0); }
 
 // a copy of an LDS struct whose every 32-bit word is wave-uniform
 template <class T>
@@ -127,106 +176,287 @@ __device__ __forceinline__ void serve_pixel(const ServeDesc &d, const DevScene &
   d.shade[pix] = res.shade;
 }
 
-__global__ void __launch_bounds__(kServeThreads)
-    serve_persistent_kernel(const ServeDesc *desc, const ServeCtl *ctl, uint64_t *done_flag,
-                            ServeState *st, uint32_t *done_cnt, uint32_t ring_mask, uint32_t gen,
-                            uint64_t idle_ticks) {
-  __shared__ ServeDesc d;
-  __shared__ uint32_t s_cmd, s_tile, s_load;
-  __shared__ uint64_t s_entry;
-  const bool leader = blockIdx.x < kServeLeaders;
-  uint64_t have = 0;  // thread 0: entry + 1 whose descriptor d holds (0: none)
-  uint64_t idle_since = wall_clock64();
+// lane 0 performs f() (an atomic, or a load of a word every lane would read
+// alike); every lane gets its value.  The region is one access with no loop,
+// entered and left by the whole wave, so the scheduler's control flow stays
+// wave-uniform: a loop around a lane-0 region lets the compiler iterate the
+// other lanes past it on their own.
+template <class F>
+__device__ __forceinline__ uint64_t by_lane0_64(uint32_t lane, F f) {
+  uint64_t v = 0;
+  if (lane == 0) v = f();
+  return lane0(v);
+}
+template <class F>
+__device__ __forceinline__ uint32_t by_lane0_32(uint32_t lane, F f) {
+  uint32_t v = 0;
+  if (lane == 0) v = f();
+  return lane0(v);
+}
+
+// a head word (read, or returned by an add): units left
+__device__ __forceinline__ bool claimable(uint64_t c) {
+  return (c & kFieldMask) < ((c >> 21) & kFieldMask);
+}
+__device__ __forceinline__ uint32_t units_left(uint64_t c) {
+  return claimable(c) ? (uint32_t)(((c >> 21) & kFieldMask) - (c & kFieldMask)) : 0u;
+}
+// the largest v over the wave
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = (uint32_t)__shfl_xor((int)v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// Move head x (read as h, exhausted) on to the entry after its own when that
+// one is published (pub): it is open, as this head has not passed it, so its
+// descriptor is stable.  Called by single lanes.
+__device__ __forceinline__ void advance_head(ServeState *st, const ServeDesc *ddesc,
+                                             uint32_t ring_mask, uint32_t x, uint64_t h,
+                                             uint64_t pub) {
+  const uint64_t ahead = (pub - (h >> 42)) & kEntryMask;
+  if (ahead < 1 || ahead >= (1ull << 21)) return;
+  const uint64_t n = ((h >> 42) + 1) & kEntryMask;
+  const uint32_t units = __hip_atomic_load(&ddesc[n & ring_mask].units, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+  atomicCAS(&st->heads[x].word, (unsigned long long)h,
+            (unsigned long long)head_word(n, band_units(units, x)));
+}
+
+// An installed chunk: tickets [lo, hi) (packed in `span`, written last) are
+// units [u0, u0 + hi - lo) of ring entry `entry`, whose descriptor is
+// desc[slot]; `done` counts its finished tickets.
+struct ChunkSlot {
+  uint64_t span;  // lo | hi << 32
+  uint64_t entry;
+  uint32_t u0, done;
+};
+
+// The publisher wave (see the header).
+__device__ void serve_publisher(const ServeDesc *desc, ServeDesc *ddesc, const ServeCtl *ctl,
+                                ServeState *st, uint32_t ring_mask, uint32_t gen,
+                                uint64_t idle_ticks) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint64_t copied = by_lane0_64(lane, [&] { return (uint64_t)dev_load64(&st->xcd[0].published); });
+  uint64_t last_busy = wall_clock64();
   for (;;) {
-    if (threadIdx.x == 0) {
-      uint32_t cmd = 0, nap = 1;
+    const uint32_t stop = by_lane0_32(lane, [&] { return sys_load32(&ctl->stop); });
+    const uint64_t hp = by_lane0_64(lane, [&] { return sys_load64(&ctl->published); });
+    if (hp > copied) {
+      for (uint64_t e = copied + 1; e <= hp; e++) {
+        // entry e's slot held an entry that has completed, which no wave reads
+        const uint64_t *src = (const uint64_t *)&desc[e & ring_mask];
+        uint64_t *dst = (uint64_t *)&ddesc[e & ring_mask];
+        if (lane < kDescWords) dst[lane] = sys_load64(src + lane);
+      }
+      // the copies are written back before the counts that announce them move
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      copied = hp;
+      if (lane < kServeHeads) atomicMax(&st->xcd[lane].published, (unsigned long long)hp);
+      last_busy = wall_clock64();
+    }
+    // heads: move exhausted ones on; any with units left means work
+    uint64_t h = 0;
+    if (lane < kServeHeads) h = dev_load64(&st->heads[lane].word);
+    if (lane < kServeHeads && !claimable(h)) advance_head(st, ddesc, ring_mask, lane, h, copied);
+    const uint64_t now = wall_clock64();
+    if (__ballot(lane < kServeHeads && claimable(h))) last_busy = now;
+    if (stop || now - last_busy > idle_ticks) {
+      if (lane < kServeHeads) atomicMax(&st->xcd[lane].exiting, stop ? 0xFFFFFFFFu : gen);
+      return;
+    }
+    __builtin_amdgcn_s_sleep(16);
+  }
+}
+
+__global__ void __launch_bounds__(kServeThreads)
+    serve_persistent_kernel(const ServeDesc *desc, ServeDesc *ddesc, const ServeCtl *ctl,
+                            uint64_t *done_flag, ServeState *st, uint32_t *done_cnt,
+                            uint32_t ring_mask, uint32_t gen, uint64_t idle_ticks,
+                            uint64_t rescan_ticks) {
+  __shared__ ServeDesc cdesc[kSlots];  // descriptors of the installed chunks' entries
+  __shared__ ChunkSlot ch[kSlots];
+  __shared__ uint32_t next_ticket, installed, n_installed, leave;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t home = blockIdx.x % kServeHeads;
+  ServeXcd *const mine = &st->xcd[home];
+  if (threadIdx.x < kSlots) {
+    ch[threadIdx.x].span = 0;
+    ch[threadIdx.x].entry = ~0ull;
+    ch[threadIdx.x].u0 = 0;
+    ch[threadIdx.x].done = 0;
+  }
+  if (threadIdx.x == 0) {
+    next_ticket = 0;
+    installed = 0;
+    n_installed = 0;
+    leave = 0;
+  }
+  __syncthreads();  // the only barrier: every wave runs on its own from here
+  if (blockIdx.x == 0 && wave == 0) {
+    serve_publisher(desc, ddesc, ctl, st, ring_mask, gen, idle_ticks);
+    return;
+  }
+  // Claim the next chunk from the heads for ticket t, the first past the
+  // newest chunk, and install it; false when the launch ends.
+  auto refill = [&](uint32_t t) -> bool {
+    const uint32_t k = lane0(n_installed) % kSlots;
+    // the slot's previous chunk must be finished (its tickets are all
+    // taken, and their waves finish without waiting for anything)
+    for (;;) {
+      const uint64_t span = lane0(ch[k].span);
+      const uint32_t n = (uint32_t)(span >> 32) - (uint32_t)span;
+      if (lane0(ch[k].done) == n) break;
+      __builtin_amdgcn_s_sleep(1);
+    }
+    uint64_t c = 0;
+    uint32_t x = 0;
+    uint32_t nap = 1;
+    bool wide = false;
+    for (;;) {
+      // own head (lane 0), the XCD's ... then every head, one per lane
+      uint64_t hl = 0;
+      const uint32_t hx = (home + lane) % kServeHeads;
+      if (lane < (wide ? kServeHeads : 1u)) hl = dev_load64(&st->heads[hx].word);
+      const uint32_t left = units_left(hl);
+      const uint32_t most = wave_max(left);
+      if (most) {
+        const uint64_t at = __ballot(left == most);
+        x = lane_at(hx, (uint32_t)__ffsll((long long)at) - 1);
+        c = by_lane0_64(lane, [&] {
+          return (uint64_t)atomicAdd(&st->heads[x].word, (unsigned long long)kChunk);
+        });
+        if (claimable(c)) break;
+        wide = false;
+        continue;  // that head ran out meanwhile: look again
+      }
+      if (!wide) {
+        wide = true;
+        continue;
+      }
+      // every head exhausted: move on the first one behind the published
+      // count (the publisher moves them on too)
+      const uint64_t pub =
+          by_lane0_64(lane, [&] { return (uint64_t)dev_load64(&mine->published); });
+      const uint64_t ahead = (pub - (hl >> 42)) & kEntryMask;
+      const uint64_t behind = __ballot(lane < kServeHeads && ahead >= 1 && ahead < (1ull << 21));
+      if (behind) {
+        const uint32_t r = (uint32_t)__ffsll((long long)behind) - 1;
+        const uint32_t bx = lane_at(hx, r);
+        const uint64_t bh = lane_at(hl, r);
+        if (lane == 0) advance_head(st, ddesc, ring_mask, bx, bh, pub);
+        wide = false;
+        continue;
+      }
+      // nothing to claim: wait for the XCD's published count to move (or
+      // kRescanUs), or for the launch to end
+      const uint64_t since = wall_clock64();
       for (;;) {
-        // only the leaders read host memory here: the stop request and the
-        // published count, passed on to the others in device memory
-        uint64_t pub = dev_load64(&st->mirror);
-        if (leader) {
-          if (sys_load32(&ctl->stop)) atomicMax(&st->exiting, 0xFFFFFFFFu);
-          const uint64_t hp = sys_load64(&ctl->published);
-          if (hp > pub) {
-            atomicMax(&st->mirror, (unsigned long long)hp);
-            pub = hp;
-          }
+        for (uint32_t j = 0; j < nap; j++) __builtin_amdgcn_s_sleep(32);
+        nap = nap < 64 ? nap * 2 : 64;
+        if (by_lane0_32(lane, [&] { return dev_load32(&mine->exiting); }) >= gen) {
+          if (lane == 0) leave = 1;
+          return false;
         }
-        if (__hip_atomic_load(&st->exiting, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= gen) {
-          cmd = 2;
-          break;
-        }
-        uint64_t h = dev_load64(&st->head);
-        if ((h & kFieldMask) < ((h >> 21) & kFieldMask)) {
-          // the entry has tiles left: claim one
-          h = atomicAdd(&st->head, 1ull);
-          const uint64_t tiles = (h >> 21) & kFieldMask, cur = h & kFieldMask;
-          if (cur < tiles) {
-            const uint64_t e = h >> 42;
-            s_entry = e;
-            s_tile = (uint32_t)cur;
-            s_load = have != e + 1;
-            have = e + 1;
-            cmd = 1;
-            break;
-          }
-          continue;  // exhausted meanwhile: look again
-        }
-        // the entry is exhausted: move the head to the next published entry
-        const uint64_t e = h >> 42;
-        const uint64_t ahead = (pub - e) & kEntryMask;
-        if (ahead >= 1 && ahead < (1ull << 21)) {
-          const uint64_t n = (e + 1) & kEntryMask;
-          const uint32_t nt = sys_load32(&desc[n & ring_mask].tiles_total);
-          atomicCAS(&st->head, (unsigned long long)h, (unsigned long long)head_word(n, nt));
-          continue;  // whoever won, claim from the new head
-        }
-        // nothing to claim
-        have = 0;
-        if (wall_clock64() - idle_since > idle_ticks) {
-          if (leader) atomicMax(&st->exiting, gen);
-          cmd = 2;
-          break;
-        }
-        for (uint32_t k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(32);
-        nap = nap < 16 ? nap * 2 : 16;
+        const uint64_t p =
+            by_lane0_64(lane, [&] { return (uint64_t)dev_load64(&mine->published); });
+        if (p != pub || wall_clock64() - since > rescan_ticks) break;
       }
-      s_cmd = cmd;
+      wide = false;
     }
-    __syncthreads();
-    if (s_cmd == 2) return;
-    if (s_load) {
-      // a valid claim holds the entry open, so its descriptor is stable
-      const uint64_t *src = (const uint64_t *)&desc[s_entry & ring_mask];
-      if (threadIdx.x < kDescWords) ((uint64_t *)&d)[threadIdx.x] = sys_load64(src + threadIdx.x);
-      __syncthreads();
+    const uint64_t ent = c >> 42;
+    const uint32_t cur = (uint32_t)(c & kFieldMask), count = (uint32_t)((c >> 21) & kFieldMask);
+    const uint32_t n = count - cur < kChunk ? count - cur : kChunk;
+    if (cur + n == count) {
+      // this claim took the band's last units: move the head on at once
+      const uint64_t pub =
+          by_lane0_64(lane, [&] { return (uint64_t)dev_load64(&mine->published); });
+      if (lane == 0) advance_head(st, ddesc, ring_mask, x, c + kChunk, pub);
     }
-    const uint32_t tiles_x = __builtin_amdgcn_readfirstlane(d.tiles_x);
-    const uint32_t tx = s_tile % tiles_x, ty = s_tile / tiles_x;
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const DevCamera cam = uniform_copy(d.cam);
-    const int32_t px = (int32_t)(tx * kServeTile + (wave & 1u) * 8u + (lane & 7u));
-    const int32_t py = (int32_t)(ty * kServeTile + (wave >> 1) * 8u + (lane >> 3));
-    if (px < cam.width && py < cam.height) {
+    if (lane0(ch[k].entry) != ent) {
+      // the chunk holds the entry open, so its descriptor is stable
+      const unsigned long long *src = (const unsigned long long *)&ddesc[ent & ring_mask];
+      if (lane < kDescWords) ((uint64_t *)&cdesc[k])[lane] = dev_load64(src + lane);
+    }
+    const uint32_t units = __builtin_amdgcn_readfirstlane(cdesc[k].units);
+    if (lane == 0) {
+      ch[k].entry = ent;
+      ch[k].u0 = band_start(units, x) + cur;
+      ch[k].done = 0;
+      ch[k].span = (uint64_t)t | ((uint64_t)(t + n) << 32);  // last: it validates the slot
+      installed = t + n;
+      n_installed = n_installed + 1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    return true;
+  };
+  for (;;) {
+    const uint32_t t = by_lane0_32(lane, [&] { return atomicAdd(&next_ticket, 1u); });
+    // the chunk holding ticket t; the slot search reads each span whole, so
+    // a slot being refilled never matches a ticket of another chunk
+    uint32_t k = kSlots;
+    for (;;) {
+      for (uint32_t j = 0; j < kSlots; j++) {
+        const uint64_t span = lane0(ch[j].span);
+        if (t - (uint32_t)span < (uint32_t)(span >> 32) - (uint32_t)span) {
+          k = j;
+          break;
+        }
+      }
+      if (k < kSlots) break;
+      if (lane0(leave)) return;
+      if (t == lane0(installed)) {
+        if (!refill(t)) return;
+        continue;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    // this ticket's packet
+    const ServeDesc &d = cdesc[k];
+    const uint64_t span = lane0(ch[k].span);
+    const uint32_t u = lane0(ch[k].u0) + (t - (uint32_t)span);
+    const uint32_t n = (uint32_t)(span >> 32) - (uint32_t)span;
+    // what publishing the chunk needs, read before this wave's count: once
+    // the count is in, the slot may be refilled
+    const uint64_t seq = lane0((uint64_t)d.seq);
+    const uint32_t total = __builtin_amdgcn_readfirstlane(d.units);
+    const uint32_t packets = __builtin_amdgcn_readfirstlane(d.packets);
+    const uint32_t slot = (uint32_t)seq & ring_mask;
+    if (u < packets) {
+      const uint32_t packets_x = __builtin_amdgcn_readfirstlane(d.packets_x);
+      const DevCamera cam = uniform_copy(d.cam);
       const DevScene scene = uniform_copy(d.scene);
-      switch (__builtin_amdgcn_readfirstlane(d.kind)) {
-        case MESO_SCENE_DENSE_U8: serve_pixel<1>(d, scene, cam, px, py); break;
-        case MESO_SCENE_BRICKMAP_U16: serve_pixel<2>(d, scene, cam, px, py); break;
-        default: serve_pixel<3>(d, scene, cam, px, py); break;
+      const int kind = (int)__builtin_amdgcn_readfirstlane(d.kind);
+      const int32_t px = (int32_t)((u % packets_x) * kServePacket + (lane & 7u));
+      const int32_t py = (int32_t)((u / packets_x) * kServePacket + (lane >> 3));
+      if (px < cam.width && py < cam.height) {
+        switch (kind) {
+          case MESO_SCENE_DENSE_U8: serve_pixel<1>(d, scene, cam, px, py); break;
+          case MESO_SCENE_BRICKMAP_U16: serve_pixel<2>(d, scene, cam, px, py); break;
+          default: serve_pixel<3>(d, scene, cam, px, py); break;
+        }
       }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      // the tile's outputs are written back before the count moves
-      const uint32_t slot = (uint32_t)s_entry & ring_mask;
-      const uint32_t old =
-          __hip_atomic_fetch_add(&done_cnt[slot], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-      if (old + 1 == d.tiles_total) {
-        __hip_atomic_store(&done_cnt[slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&done_flag[slot], (uint64_t)d.seq, __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_SYSTEM);
+    // count the ticket: this wave's stores are ordered before the count
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    const uint32_t finished = by_lane0_32(lane, [&] { return atomicAdd(&ch[k].done, 1u); }) + 1;
+    if (finished == n) {
+      // the chunk's last ticket: every wave's stores for it are ordered
+      // before this point; write them back, then move the entry's counter
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+      uint32_t *const cnt = &done_cnt[slot * kCntStride];
+      const uint32_t old = by_lane0_32(lane, [&] {
+        return __hip_atomic_fetch_add(cnt, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      });
+      if (old + n == total && lane == 0) {
+        __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&done_flag[slot], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
       }
-      idle_since = wall_clock64();
     }
   }
 }
@@ -238,7 +468,7 @@ struct ReqPayload {
   uint32_t flags;
   int32_t index_bytes;
   uint64_t pixels;
-  uint32_t tiles_x;
+  uint32_t packets_x;
 };
 
 static uint64_t request_bytes(uint64_t pixels, int32_t index_bytes) {
@@ -251,8 +481,9 @@ static uint64_t request_bytes(uint64_t pixels, int32_t index_bytes) {
 // from client threads, hence the atomics.
 class HipServeDevice final : public meso_serve::ServeDevice<ReqPayload> {
  public:
-  HipServeDevice(meso_context *ctx, uint32_t ring, int grid, uint64_t idle_ticks)
-      : ctx_(ctx), ring_(ring), grid_(grid), idle_ticks_(idle_ticks) {}
+  HipServeDevice(meso_context *ctx, uint32_t ring, int grid, uint64_t idle_ticks,
+                 uint64_t rescan_ticks)
+      : ctx_(ctx), ring_(ring), grid_(grid), idle_ticks_(idle_ticks), rescan_ticks_(rescan_ticks) {}
   ~HipServeDevice() override { release(); }
 
   meso_status init(uint64_t arena_bytes) {
@@ -271,7 +502,8 @@ class HipServeDevice final : public meso_serve::ServeDevice<ReqPayload> {
     MESO_HIP(hipHostGetDevicePointer((void **)&d_desc_, desc_, 0));
     MESO_HIP(hipHostGetDevicePointer((void **)&d_ctl_, ctl_, 0));
     MESO_HIP(hipHostGetDevicePointer((void **)&d_done_flag_, done_flag_, 0));
-    const uint64_t state_bytes = sizeof(ServeState) + sizeof(uint32_t) * ring_;
+    // device state: ServeState, the completion counters, the device ring
+    const uint64_t state_bytes = ddesc_offset() + sizeof(ServeDesc) * ring_;
     MESO_HIP(hipMalloc((void **)&state_, state_bytes));
     MESO_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     MESO_HIP(hipEventCreate(&ev_start_));
@@ -284,12 +516,13 @@ class HipServeDevice final : public meso_serve::ServeDevice<ReqPayload> {
 
   uint8_t *arena() const { return arena_; }
 
-  void publish(uint64_t seq, uint32_t slot, uint64_t arena_off, uint32_t tiles,
+  void publish(uint64_t seq, uint32_t slot, uint64_t arena_off, uint32_t packets,
                const ReqPayload &p) override {
     ServeDesc d{};
     d.seq = seq;
-    d.tiles_total = tiles;
-    d.tiles_x = p.tiles_x;
+    d.packets = packets;
+    d.packets_x = p.packets_x;
+    d.units = std::max(packets, kServeHeads);  // at least one unit on every head
     d.kind = p.scene->kind;
     d.flags = p.flags;
     d.index_bytes = p.index_bytes;
@@ -334,9 +567,10 @@ class HipServeDevice final : public meso_serve::ServeDevice<ReqPayload> {
     hipSetDevice(ctx_->device);
     hipEventRecord(ev_start_, stream_);
     hipLaunchKernelGGL(serve_persistent_kernel, dim3((uint32_t)grid_), dim3(kServeThreads), 0,
-                       stream_, d_desc_, d_ctl_, d_done_flag_, (ServeState *)state_,
+                       stream_, d_desc_, (ServeDesc *)((uint8_t *)state_ + ddesc_offset()),
+                       d_ctl_, d_done_flag_, (ServeState *)state_,
                        (uint32_t *)((uint8_t *)state_ + sizeof(ServeState)), ring_ - 1, gen_,
-                       idle_ticks_);
+                       idle_ticks_, rescan_ticks_);
     const hipError_t e = hipGetLastError();
     hipEventRecord(ev_end_, stream_);
     if (e != hipSuccess) {
@@ -372,6 +606,9 @@ class HipServeDevice final : public meso_serve::ServeDevice<ReqPayload> {
     std::lock_guard<std::mutex> lk(err_mu_);
     error_ = m;
   }
+  uint64_t ddesc_offset() const {
+    return sizeof(ServeState) + sizeof(uint32_t) * kCntStride * ring_;
+  }
   void release() {
     shutdown();
     hipSetDevice(ctx_->device);
@@ -395,7 +632,7 @@ class HipServeDevice final : public meso_serve::ServeDevice<ReqPayload> {
   meso_context *ctx_;
   uint32_t ring_;
   int grid_;
-  uint64_t idle_ticks_;
+  uint64_t idle_ticks_, rescan_ticks_;
   uint8_t *arena_ = nullptr;
   ServeDesc *desc_ = nullptr, *d_desc_ = nullptr;
   ServeCtl *ctl_ = nullptr, *d_ctl_ = nullptr;
@@ -439,25 +676,27 @@ meso_status meso_serve_start(meso_context *ctx, const meso_serve_config *cfg, me
              cfg->workgroups_per_cu);
   MESO_CHECK(cfg->idle_exit_us >= 0 && cfg->idle_exit_us <= 1000000, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
              "idle_exit_us %d not in 0..1000000", cfg->idle_exit_us);
-  MESO_CHECK(cfg->tile == 0 || cfg->tile == kServeTile, MESO_ERR_NOT_SUPPORTED,
-             "tile edge must be %d", kServeTile);
+  MESO_CHECK(cfg->tile == 0 || cfg->tile == kServePacket, MESO_ERR_NOT_SUPPORTED,
+             "packet edge must be %d", kServePacket);
   MESO_HIP(hipSetDevice(ctx->device));
   int per_cu = 0;
   MESO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_persistent_kernel,
                                                          kServeThreads, 0));
   if (per_cu < 1) per_cu = 1;
   if (cfg->workgroups_per_cu > 0 && cfg->workgroups_per_cu < per_cu) per_cu = cfg->workgroups_per_cu;
-  const int grid = std::max<int>(per_cu * ctx->props.multiProcessorCount, (int)kServeLeaders);
+  const int grid = std::max<int>(per_cu * ctx->props.multiProcessorCount, (int)kServeHeads);
   int clock_khz = 0;
   MESO_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, ctx->device));
   const int idle_us = cfg->idle_exit_us ? cfg->idle_exit_us : 500;
-  const uint64_t idle_ticks = (uint64_t)idle_us * (uint64_t)(clock_khz > 0 ? clock_khz : 100000) / 1000;
+  const uint64_t ticks_per_ms = (uint64_t)(clock_khz > 0 ? clock_khz : 100000);
+  const uint64_t idle_ticks = (uint64_t)idle_us * ticks_per_ms / 1000;
+  const uint64_t rescan_ticks = (uint64_t)kRescanUs * ticks_per_ms / 1000;
   uint32_t ring = 1;
   while (ring < (uint32_t)cfg->max_inflight) ring <<= 1;
   auto s = std::make_unique<meso_server>();
   s->ctx = ctx;
   s->cfg = *cfg;
-  s->dev = std::make_unique<HipServeDevice>(ctx, ring, grid, idle_ticks);
+  s->dev = std::make_unique<HipServeDevice>(ctx, ring, grid, idle_ticks, rescan_ticks);
   const meso_status st = s->dev->init(cfg->hbm_budget_bytes);
   if (st != MESO_OK) return st;
   meso_serve::CoreConfig cc;
@@ -500,13 +739,17 @@ meso_status meso_serve_submit(meso_server *s, meso_scene *scene, const meso_came
   const uint64_t nvox = (uint64_t)scene->n * scene->n * scene->n;
   r.index_bytes = nvox > 0x7FFFFFFFull ? 8 : 4;
   r.pixels = (uint64_t)cam->width * (uint64_t)cam->height;
-  r.tiles_x = (uint32_t)((cam->width + kServeTile - 1) / kServeTile);
-  const uint32_t tiles = r.tiles_x * (uint32_t)((cam->height + kServeTile - 1) / kServeTile);
+  r.packets_x = (uint32_t)((cam->width + kServePacket - 1) / kServePacket);
+  const uint64_t packets =
+      (uint64_t)r.packets_x * (uint64_t)((cam->height + kServePacket - 1) / kServePacket);
+  MESO_CHECK(packets < kMaxPackets, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
+             "request of %dx%d pixels exceeds the %u 8x8 packets of one served request",
+             cam->width, cam->height, kMaxPackets - 1);
   const uint64_t bytes = request_bytes(r.pixels, r.index_bytes);
   MESO_CHECK(bytes <= s->cfg.hbm_budget_bytes, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
              "request needs %llu bytes, budget is %llu", (unsigned long long)bytes,
              (unsigned long long)s->cfg.hbm_budget_bytes);
-  const meso_status st = s->core->submit(r, bytes, tiles, r.pixels, ticket);
+  const meso_status st = s->core->submit(r, bytes, (uint32_t)packets, r.pixels, ticket);
   if (st == MESO_ERR_BUSY)
     return fail(MESO_ERR_BUSY, "pending queue full (%d)", s->cfg.max_pending);
   if (st != MESO_OK) return fail(st, "server stopped or failed: %s", s->core->error().c_str());

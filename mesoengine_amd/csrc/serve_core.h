@@ -94,8 +94,8 @@ class ServeDevice {
  public:
   virtual ~ServeDevice() = default;
   // Make entry `seq` visible to the device: its payload, the arena offset of
-  // its outputs and its tile count.  Called in increasing seq order.
-  virtual void publish(uint64_t seq, uint32_t slot, uint64_t arena_off, uint32_t tiles,
+  // its outputs and its work-item count.  Called in increasing seq order.
+  virtual void publish(uint64_t seq, uint32_t slot, uint64_t arena_off, uint32_t items,
                        const Payload &p) = 0;
   // true once the device has completed entry `seq` (held in `slot`)
   virtual bool completed(uint32_t slot, uint64_t seq) = 0;
@@ -125,7 +125,7 @@ struct Ticket {
   uint64_t id = 0;
   Payload payload{};
   uint64_t bytes = 0;
-  uint32_t tiles = 0;
+  uint32_t items = 0;  // work items of the device (the server's 8x8 packets)
   uint64_t pixels = 0;
   uint64_t seq = 0;
   uint32_t slot = 0;
@@ -150,7 +150,7 @@ class ServeCore {
 
   // MESO_OK with *ticket set; MESO_ERR_BUSY when the pending queue is full;
   // MESO_ERR_RUNTIME after stop or a device error.
-  meso_status submit(const Payload &p, uint64_t bytes, uint32_t tiles, uint64_t pixels,
+  meso_status submit(const Payload &p, uint64_t bytes, uint32_t items, uint64_t pixels,
                      uint64_t *ticket) {
     {
       std::lock_guard<std::mutex> lk(mu_);
@@ -163,7 +163,7 @@ class ServeCore {
       t.id = next_ticket_++;
       t.payload = p;
       t.bytes = bytes;
-      t.tiles = tiles;
+      t.items = items;
       t.pixels = pixels;
       t.t_submit = Clock::now();
       *ticket = t.id;
@@ -313,7 +313,7 @@ class ServeCore {
         stats_.bytes_in_use += Arena::granule(t.bytes);
         stats_.peak_bytes_in_use = std::max(stats_.peak_bytes_in_use, stats_.bytes_in_use);
         stats_.primary_rays += t.pixels;
-        dev_->publish(t.seq, slot, off, t.tiles, t.payload);
+        dev_->publish(t.seq, slot, off, t.items, t.payload);
         inflight_.push_back(t.id);
         pending_.pop_front();
       }

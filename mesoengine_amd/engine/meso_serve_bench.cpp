@@ -303,7 +303,7 @@ int main(int argc, char **argv) {
   cfg.max_pending = (int32_t)reqs.size() + 8;
   cfg.max_inflight = inflight;
   cfg.workgroups_per_cu = wg_per_cu;
-  cfg.tile = 16;
+  cfg.tile = 8;
   const bool timed = !capacity;
   // warm both paths once on the first few requests
   {

@@ -290,13 +290,14 @@ class Event:
 class Server:
     """The continuous-batching server: requests are admitted by the HBM
     budget into `max_inflight` ring entries that one persistent kernel of
-    `workgroups_per_cu` workgroups per CU (0: all that fit) draws tiles from."""
+    `workgroups_per_cu` workgroups of 4 waves per CU (0: all that fit) draws
+    8x8-pixel packets from."""
 
     def __init__(self, ctx, budget_bytes, max_pending=256, max_inflight=64, workgroups_per_cu=0,
                  idle_exit_us=0):
         self.ctx = ctx
         cfg = L.ServeConfig(C.sizeof(L.ServeConfig), budget_bytes, max_pending, max_inflight,
-                            workgroups_per_cu, 16, idle_exit_us)
+                            workgroups_per_cu, 8, idle_exit_us)
         h = C.c_void_p()
         check(ctx.lib.meso_serve_start(ctx.h, byref(cfg), C.byref(h)))
         self.h = h

@@ -53,14 +53,14 @@ class FakeDevice final : public meso_serve::ServeDevice<Payload> {
   }
   ~FakeDevice() override { shutdown(); }
 
-  // bytes of a request are carried in `tiles` for this fake (tiles * 64)
-  void publish(uint64_t seq, uint32_t slot, uint64_t off, uint32_t tiles,
+  // bytes of a request are carried in `items` for this fake (items * 64)
+  void publish(uint64_t seq, uint32_t slot, uint64_t off, uint32_t items,
                const Payload &p) override {
     EXPECT(busy_[slot].exchange(seq) == 0);  // slot free
     EXPECT(seq == last_seq_ + 1);             // in order
     last_seq_ = seq;
     std::lock_guard<std::mutex> lk(mu_);
-    queue_.push_back(Entry{seq, slot, off, (uint64_t)tiles * 64, p.tag});
+    queue_.push_back(Entry{seq, slot, off, (uint64_t)items * 64, p.tag});
     cv_.notify_one();
   }
   bool completed(uint32_t slot, uint64_t seq) override {

@@ -55,7 +55,7 @@ def test_lookback_wait_gives_up_and_is_reported(ctx):
 def test_impossible_allocations_leave_the_context_usable(ctx):
     h = C.c_void_p()
     assert ctx.lib.meso_buffer_create(ctx.h, 1 << 50, None, C.byref(h)) == L.ERR_OOM
-    cfg = L.ServeConfig(C.sizeof(L.ServeConfig), 1 << 50, 8, 8, 0, 16, 0)
+    cfg = L.ServeConfig(C.sizeof(L.ServeConfig), 1 << 50, 8, 8, 0, 8, 0)
     assert ctx.lib.meso_serve_start(ctx.h, L.byref(cfg), C.byref(h)) == L.ERR_OOM
     # still usable
     sc = ctx.scene_generate(L.GEN_SPHERES, L.SCENE_DENSE_U8, 128, 1, 8)
