@@ -36,6 +36,16 @@ def dirty_chunks(n, frame, seed, frac=0.01):
     return np.stack([flat % nc, (flat // nc) % nc, flat // (nc * nc)], 1).astype(np.int32)
 
 
+def frame_edit(n, frame, seed):
+    """The dirty chunks and edit seed of frame `frame`.  Frames come in
+    pairs that apply the same edit: an edit toggles occupancy inside its
+    spheres (solid -> empty, empty -> solid), so the second frame of a pair
+    restores the first's occupancy and the scene does not drift over a run;
+    every frame still edits and remeshes 1% of the chunks."""
+    k = frame // 2
+    return dirty_chunks(n, k, seed), seed * 7919 + k
+
+
 def frame_bytes(cnt, index_bytes=8):
     """Algorithmic bytes of one rendered frame (the path's per-ray byte
     model): 4 B per top-level and node pointer read, 64 B per non-empty brick
@@ -97,20 +107,26 @@ class _Frames:
                                                         L.byref(t)))
 
     def count(self):
+        """step counts of one render of the current scene"""
+        self.tg.counters.zero()
         R.raymarch(self.ctx, self.scene, self.cam, self.flags | L.RAY_COUNT, self.sh, self.tg)
         self.ctx.sync()
         return self.tg.counts()
 
+    def _edits(self, frames, first):
+        plan = [frame_edit(self.cfg["n"], first + f, self.seed) for f in range(frames)]
+        lists = [self.ctx.upload(xyz.ravel()) for xyz, _ in plan]
+        self.ctx.sync()
+        return lists, [s for _, s in plan]
+
     def run(self, frames, first=0):
         """Time `frames` frames; returns per-frame (edit, remesh, render) ms and quads."""
-        lists = [self.ctx.upload(dirty_chunks(self.cfg["n"], first + f, self.seed).ravel())
-                 for f in range(frames)]
-        self.ctx.sync()
+        lists, seeds = self._edits(frames, first)
         out = []
         e = self.ev
         for f in range(frames):
             e[0].record()
-            R.edit_chunks(self.scene, lists[f], self.n_dirty, self.seed * 7919 + first + f)
+            R.edit_chunks(self.scene, lists[f], self.n_dirty, seeds[f])
             e[1].record()
             R.mesh_scene_chunks(self.ctx, self.scene, lists[f], self.n_dirty, L.MESH_GREEDY,
                                 self.mt)
@@ -128,14 +144,12 @@ class _Frames:
         edit runs on the context stream; its remesh runs on `stream2` beside
         its render (both only read the edited scene); the next frame's edit
         waits for both.  Returns ms per frame."""
-        lists = [self.ctx.upload(dirty_chunks(self.cfg["n"], first + f, self.seed).ravel())
-                 for f in range(frames)]
-        self.ctx.sync()
+        lists, seeds = self._edits(frames, first)
         L.check(self.ctx.lib.meso_stream_sync(self.ctx.h, stream2))
         a, b, ev_edit, ev_mesh = (R.Event(self.ctx) for _ in range(4))
         a.record()
         for f in range(frames):
-            R.edit_chunks(self.scene, lists[f], self.n_dirty, self.seed * 7919 + first + f)
+            R.edit_chunks(self.scene, lists[f], self.n_dirty, seeds[f])
             ev_edit.record()
             L.check(self.ctx.lib.meso_stream_wait_event(self.ctx.h, stream2, ev_edit.h))
             R.mesh_scene_chunks(self.ctx, self.scene, lists[f], self.n_dirty, L.MESH_GREEDY,
@@ -146,6 +160,18 @@ class _Frames:
         b.record()
         b.sync()
         return a.elapsed_ms(b) / frames
+
+    def count_frames(self, frames, first=0):
+        """The step counts summed over `frames` frames (edits applied as in
+        the timed frames, renders with the counters on, untimed): replayed
+        from the same occupancy, they are the counts of the timed frames."""
+        lists, seeds = self._edits(frames, first)
+        self.tg.counters.zero()
+        for f in range(frames):
+            R.edit_chunks(self.scene, lists[f], self.n_dirty, seeds[f])
+            R.raymarch(self.ctx, self.scene, self.cam, self.flags | L.RAY_COUNT, self.sh, self.tg)
+        self.ctx.sync()
+        return self.tg.counts()
 
     def render_alone(self, k):
         """k renders back to back on the context stream (the stream the
@@ -182,8 +208,8 @@ def _under_load(ctx, fr, scenes, reqs, budget_bytes, frames, rays, log, label):
 
     def serve():
         try:
-            # the server's persistent kernel on a quarter of the wave slots
-            # (2 workgroups of 4 waves per CU), so the render keeps the rest
+            # the server's persistent kernel on 8 wave slots per CU (2
+            # workgroups of 4 waves), so the render keeps the rest
             result["lat"], result["makespan"], result["stats"] = serving.served_run(
                 ctx, scenes, reqs, sh, budget_bytes, True, workgroups_per_cu=2)
         except Exception as ex:  # surfaced after join
@@ -229,6 +255,8 @@ def headline(ctx, steps=20, warmup=3, seed=5, log=print):
         rays = cnt["rays"] + cnt["shadow_rays"] + cnt["ao_rays"]
         alg = frame_bytes(cnt)
         import ctypes as C
+        steps += steps % 2  # whole pairs: the timed frames end on the start occupancy
+        warmup += warmup % 2
         s2 = C.c_void_p()
         L.check(ctx.lib.meso_stream_create(ctx.h, C.byref(s2)))
         try:
@@ -236,6 +264,8 @@ def headline(ctx, steps=20, warmup=3, seed=5, log=print):
             ms = fr.run_overlapped(steps, s2, first=0)
         finally:
             L.check(ctx.lib.meso_stream_destroy(ctx.h, s2))
+        timed = fr.count_frames(steps, first=0)
+        rays_timed = timed["rays"] + timed["shadow_rays"] + timed["ao_rays"]
         render_ms = fr.render_alone(steps)
         split = _summary(fr.run(min(steps, 8), first=1_000))
         remesh_bytes = 65536 * fr.n_dirty + 4 * split["quads_per_frame"] + 8 * fr.n_dirty
@@ -243,8 +273,10 @@ def headline(ctx, steps=20, warmup=3, seed=5, log=print):
             workload="2048^3 two-level caves (u16, 8^3 bricks under 16^3-brick nodes), 3840x2160 "
                      "primary + shadow + 4 AO rays per hit; 1% of 32^3 chunks edited and "
                      "greedy-remeshed per frame",
-            scene=fr.info, dirty_chunks=fr.n_dirty, counts=cnt, rays_per_frame=rays,
-            render_tune=tune, ms_per_frame=ms, mrays=rays / (ms * 1e-3) / 1e6,
+            scene=fr.info, dirty_chunks=fr.n_dirty, counts=cnt,
+            rays_per_frame=int(round(rays_timed / steps)), rays_start_state=rays,
+            render_tune=tune, frames=steps, rays_timed_frames=rays_timed,
+            ms_per_frame=ms, mrays=rays_timed / steps / (ms * 1e-3) / 1e6,
             render_ms=render_ms, render_mrays=rays / (render_ms * 1e-3) / 1e6,
             render_alg_bytes=alg, render_alg_gbs=alg / (render_ms * 1e-3) / 1e9,
             split_one_stream=split,
