@@ -380,44 +380,68 @@ __device__ __forceinline__ uint32_t scene_brick_cell(const DevScene &s, int32_t 
          256u * ((uint32_t)bz & 15u);
 }
 
-// one workgroup of 512 lanes per (chunk, brick of the chunk); lane = voxel
+// One workgroup of 512 lanes per chunk (lane = voxel of a brick), walking the
+// chunk's allocated bricks: a launch of one workgroup per brick spent its
+// time dispatching workgroups, two thirds of which found no brick allocated.
+// The first wave looks up the 64 bricks' cells and slots at once (one lane
+// each) and lists the allocated ones; each brick's voxels are loaded while
+// the one before is folded.
 __global__ void __launch_bounds__(512) edit_chunks_kernel(DevScene s, int32_t kind,
                                                           const int32_t *xyz, uint32_t seed,
                                                           uint64_t *masks, uint64_t *blocks,
-                                                          uint16_t *voxels, uint64_t n_blocks) {
-  const uint64_t bid = block_id();
-  if (bid >= n_blocks) return;
-  const uint64_t chunk = bid >> 6;
-  const uint32_t b = (uint32_t)bid & 63u;
+                                                          uint16_t *voxels, uint64_t n_chunks) {
+  __shared__ uint32_t live_b[64], live_cell[64], live_slot[64];
+  __shared__ uint32_t n_live;
+  const uint64_t chunk = block_id();
+  if (chunk >= n_chunks) return;
   const int32_t nc = (s.n + 31) >> 5;  // a partial last chunk counts
   const int32_t cx = xyz[3 * chunk], cy = xyz[3 * chunk + 1], cz = xyz[3 * chunk + 2];
   if (cx < 0 || cy < 0 || cz < 0 || cx >= nc || cy >= nc || cz >= nc) return;
-  const int32_t bx = cx * 4 + (int32_t)(b & 3u), by = cy * 4 + (int32_t)((b >> 2) & 3u),
-                bz = cz * 4 + (int32_t)(b >> 4);
-  const uint32_t cell = scene_brick_cell(s, kind, bx, by, bz);
-  if (cell == kEmptySlot) return;  // uniform over the workgroup
-  const uint32_t slot = s.slots[cell];
-  if (slot == kEmptySlot) return;  // uniform over the workgroup
+  const uint32_t i = threadIdx.x;
+  if (i < 64u) {
+    const int32_t bx = cx * 4 + (int32_t)(i & 3u), by = cy * 4 + (int32_t)((i >> 2) & 3u),
+                  bz = cz * 4 + (int32_t)(i >> 4);
+    const uint32_t cell = scene_brick_cell(s, kind, bx, by, bz);
+    const uint32_t slot = cell == kEmptySlot ? kEmptySlot : s.slots[cell];
+    const unsigned long long live = __ballot(slot != kEmptySlot);
+    if (slot != kEmptySlot) {
+      const uint32_t k = (uint32_t)__popcll(live & ((1ull << i) - 1ull));  // in brick order
+      live_b[k] = i;
+      live_cell[k] = cell;
+      live_slot[k] = slot;
+    }
+    if (i == 0) n_live = (uint32_t)__popcll(live);
+  }
+  __syncthreads();
+  const uint32_t n = n_live;
+  if (n == 0) return;  // uniform over the workgroup
   const uint32_t h0 = nhash32(seed ^ nhash32((uint32_t)cx + ((uint32_t)cy + (uint32_t)cz * (uint32_t)nc) * (uint32_t)nc));
   const uint32_t h1 = nhash32(h0), h2 = nhash32(h1), h3 = nhash32(h2);
   const int64_t sx = (int64_t)cx * 8192 + (h0 & 0x1FFFu), sy = (int64_t)cy * 8192 + (h1 & 0x1FFFu),
                 sz = (int64_t)cz * 8192 + (h2 & 0x1FFFu);
   const int64_t r = 4 * 256 + (int64_t)(h3 % (8u * 256u));
-  const uint32_t i = threadIdx.x;
   const int32_t lx = (int32_t)(i & 7u), ly = (int32_t)((i >> 3) & 7u), lz = (int32_t)(i >> 6);
-  const int64_t dx = (int64_t)(bx * 8 + lx) * 256 + 128 - sx;
-  const int64_t dy = (int64_t)(by * 8 + ly) * 256 + 128 - sy;
-  const int64_t dz = (int64_t)(bz * 8 + lz) * 256 + 128 - sz;
-  uint16_t *v = voxels + (uint64_t)slot * 512u + i;
-  uint16_t val = *v;
-  if (dx * dx + dy * dy + dz * dz <= r * r) {
-    val = val ? (uint16_t)0 : (uint16_t)7;
-    *v = val;
+  uint16_t next = voxels[(uint64_t)live_slot[0] * 512u + i];
+  for (uint32_t k = 0; k < n; k++) {
+    const uint32_t b = live_b[k], cell = live_cell[k], slot = live_slot[k];
+    uint16_t val = next;
+    // the next brick's voxels are in flight meanwhile (bricks own their slots)
+    if (k + 1 < n) next = voxels[(uint64_t)live_slot[k + 1] * 512u + i];
+    const int32_t bx = cx * 4 + (int32_t)(b & 3u), by = cy * 4 + (int32_t)((b >> 2) & 3u),
+                  bz = cz * 4 + (int32_t)(b >> 4);
+    const int64_t dx = (int64_t)(bx * 8 + lx) * 256 + 128 - sx;
+    const int64_t dy = (int64_t)(by * 8 + ly) * 256 + 128 - sy;
+    const int64_t dz = (int64_t)(bz * 8 + lz) * 256 + 128 - sz;
+    if (dx * dx + dy * dy + dz * dz <= r * r) {
+      val = val ? (uint16_t)0 : (uint16_t)7;
+      voxels[(uint64_t)slot * 512u + i] = val;
+    }
+    const unsigned long long m = __ballot(val != 0);
+    if ((i & 63u) == 0) masks[(uint64_t)slot * 8u + (uint32_t)lz] = (uint64_t)m;
+    const uint64_t sub = brick_blocks2_from_planes((uint64_t)m);
+    if (i == 0) blocks[cell] = sub;
+    __syncthreads();  // the fold's LDS planes are written again for the next brick
   }
-  const unsigned long long m = __ballot(val != 0);
-  if ((i & 63u) == 0) masks[(uint64_t)slot * 8u + (uint32_t)lz] = (uint64_t)m;
-  const uint64_t sub = brick_blocks2_from_planes((uint64_t)m);
-  if (i == 0) blocks[cell] = sub;
 }
 
 }  // namespace meso
@@ -655,10 +679,9 @@ meso_status meso_scene_edit_chunks(meso_scene *scene, const int32_t *d_chunk_xyz
   if (n_chunks == 0) return MESO_OK;
   hipStream_t s = stream_of(scene->ctx, stream);
   MESO_HIP(hipSetDevice(scene->ctx->device));
-  const uint64_t n_blocks = (uint64_t)n_chunks * 64u;
-  hipLaunchKernelGGL(edit_chunks_kernel, grid_1d(n_blocks), dim3(512), 0, s, scene->dev(),
-                     scene->kind, d_chunk_xyz, seed, scene->masks, scene->blocks, scene->voxels,
-                     n_blocks);
+  hipLaunchKernelGGL(edit_chunks_kernel, grid_1d((uint64_t)n_chunks), dim3(512), 0, s,
+                     scene->dev(), scene->kind, d_chunk_xyz, seed, scene->masks, scene->blocks,
+                     scene->voxels, (uint64_t)n_chunks);
   MESO_HIP(hipGetLastError());
   return MESO_OK;
 }
