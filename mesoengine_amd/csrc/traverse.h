@@ -41,16 +41,12 @@ __device__ __forceinline__ void ray_setup(Ray &r, const float o[3], const float 
   for (int a = 0; a < 3; a++) {
     r.o[a] = o[a];
     r.d[a] = d[a];
-    if (d[a] > 0.0f) {
-      r.step[a] = 1;
-      r.inv[a] = 1.0f / d[a];
-    } else if (d[a] < 0.0f) {
-      r.step[a] = -1;
-      r.inv[a] = 1.0f / d[a];
-    } else {
-      r.step[a] = 0;
-      r.inv[a] = 0.0f;
-    }
+    // one division per axis, kept where d != 0: written per branch, the
+    // divide sequence is emitted twice and lanes of both signs run both
+    const float inv = 1.0f / d[a];
+    const bool pos = d[a] > 0.0f, neg = d[a] < 0.0f;
+    r.step[a] = pos ? 1 : (neg ? -1 : 0);
+    r.inv[a] = (pos || neg) ? inv : 0.0f;
     r.pos[a] = r.step[a] > 0 ? 1 : 0;
     r.posmask |= (r.step[a] > 0 ? 1u : 0u) << a;
     r.negmask |= (r.step[a] < 0 ? 1u : 0u) << a;
@@ -86,6 +82,21 @@ __device__ __forceinline__ int64_t lin3(int64_t x, int64_t y, int64_t z, int64_t
 // every product below 2^32.
 __device__ __forceinline__ uint32_t lin3u(uint32_t x, uint32_t y, uint32_t z, uint32_t n) {
   return x + __umul24(y + __umul24(z, n), n);
+}
+
+// The median of a, b, c: for lo <= hi, med3_i32(x, lo, hi) clamps x into
+// [lo, hi] in one instruction (the compiler cannot see lo <= hi).
+__device__ __forceinline__ int32_t med3_i32(int32_t a, int32_t b, int32_t c) {
+  int32_t m;
+  asm("v_med3_i32 %0, %1, %2, %3" : "=v"(m) : "v"(a), "v"(b), "v"(c));
+  return m;
+}
+
+// (1 << k) - 1 for k in [0, 31], one instruction
+__device__ __forceinline__ int32_t low_mask(int32_t k) {
+  int32_t m;
+  asm("v_bfm_b32 %0, %1, 0" : "=v"(m) : "v"(k));
+  return m;
 }
 
 // One step at level 2^sh (sh = 0 voxel, 3 brick, 5 and 7 empty blocks): leave
@@ -130,19 +141,21 @@ __device__ __forceinline__ bool advance(const Ray &r, const DevScene &s, int32_t
     w.face = face;
     return true;
   }
-  const int32_t span = (1 << sh) - 1;
+  const int32_t span = low_mask(sh);
 #pragma unroll
   for (int b = 0; b < 3; b++) {
     float dt = r.d[b] * t;
     float p = r.o[b] + dt;
     int32_t c = (int32_t)__builtin_floorf(p);
+    // clamp into the block [lo, hi], then not below the cell when stepping
+    // up or above it when stepping down: the cell lies in [lo, hi], so that
+    // is one clamp into [cell or lo, cell or hi] (a single med3)
     const int32_t lo = bc[b] << sh;
     int32_t hi = lo + span;
     hi = hi > n - 1 ? n - 1 : hi;
-    c = c < lo ? lo : c;
-    c = c > hi ? hi : c;
-    c = (r.step[b] > 0 && c < w.cell[b]) ? w.cell[b] : c;
-    c = (r.step[b] < 0 && c > w.cell[b]) ? w.cell[b] : c;
+    const int32_t cl = r.step[b] > 0 ? w.cell[b] : lo;
+    const int32_t ch = r.step[b] < 0 ? w.cell[b] : hi;
+    c = med3_i32(c, cl, ch);
     w.cell[b] = b == ax ? na : c;
   }
   w.t = t;
@@ -194,15 +207,21 @@ __device__ __forceinline__ bool occ_bit(const uint32_t *bits, uint32_t i) {
   return (bits[i >> 5] >> (i & 31u)) & 1u;
 }
 
-// Cached answers of the coarse levels for the cell a walk is in: packed keys
-// of the 128^3 block, 32^3 block and brick last looked up, and what they held.
+// Cached answers of the coarse levels for the cell a walk is in.  The valid
+// levels always form a prefix (the 128^3 block, then the 32^3 block, then
+// the brick), and each valid level's block holds the walk's previous cell.
+// So the cell is still in a valid level's block of edge 2^k exactly when dv,
+// the previous step's (x ^ x') | (y ^ y') | (z ^ z'), is below 2^k, and
+// max(dv, vmask) >= 2^k, with vmask one below the edge of the finest valid
+// level (255: none valid), says that level k must be looked up.
 struct WalkCache {
-  int32_t c128, c32, cb;
+  uint32_t dv, vmask;
   bool e128, e32;
   uint32_t node, slot;
   uint64_t b32, sub;  // b32: allocated bricks of the 32^3 block; sub: 2^3-block bits of the brick
   __device__ __forceinline__ void reset() {
-    c128 = c32 = cb = -1;
+    dv = 0;
+    vmask = 255u;
     e128 = e32 = false;
     node = slot = kEmpty;
     b32 = sub = 0;
@@ -241,11 +260,8 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkSt
     // answer when the cell left its block, then step at the coarsest empty
     // level, or stop at a solid voxel
     const uint32_t x = (uint32_t)w.cell[0], y = (uint32_t)w.cell[1], z = (uint32_t)w.cell[2];
-    const int32_t k128 = (int32_t)((x >> 7) | ((y >> 7) << 10) | ((z >> 7) << 20));
-    if (k128 != c.c128) {
-      c.c128 = k128;
-      c.c32 = -1;
-      c.cb = -1;
+    const uint32_t q = c.dv > c.vmask ? c.dv : c.vmask;
+    if (q >= 128u) {
       if constexpr (KIND == 3) {
         if (COUNT) cnt.n_top++;
         c.node = s.top[lin3u(x >> 7, y >> 7, z >> 7, n128)];
@@ -255,18 +271,13 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkSt
         c.e128 = !occ_bit(s.occ128, lin3u(x >> 7, y >> 7, z >> 7, n128));
       }
     }
-    const int32_t k32 = (int32_t)((x >> 5) | ((y >> 5) << 10) | ((z >> 5) << 20));
-    if (!c.e128 && k32 != c.c32) {
-      c.c32 = k32;
-      c.cb = -1;
+    if (!c.e128 && q >= 32u) {
       if (COUNT) cnt.n_coarse++;
       c.b32 = s.bricks32[lin3u(x >> 5, y >> 5, z >> 5, n32)];
       c.e32 = c.b32 == 0;
     }
     const bool coarse = c.e128 || c.e32;
-    const int32_t kb = (int32_t)((x >> 3) | ((y >> 3) << 11) | ((z >> 3) << 22));
-    if (!coarse && kb != c.cb) {
-      c.cb = kb;
+    if (!coarse && q >= 8u) {
       if (COUNT) {
         if constexpr (KIND == 3)
           cnt.n_node++;
@@ -286,24 +297,29 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkSt
       }
     }
     // inside a non-empty brick: step over its empty 4^3 and 2^3 blocks
+    // (computed on every step without a branch: outside a brick c.sub is
+    // stale and e4, e2 are not used)
     const bool in_brick = !coarse && c.slot != kEmpty;
-    bool e4 = false, e2 = false;
-    if (in_brick) {
-      const uint32_t lx = x & 7u, ly = y & 7u, lz = z & 7u;
-      e4 = ((c.sub >> (2u * (lx >> 2) + 8u * (ly >> 2) + 32u * (lz >> 2))) & 0x00330033ull) == 0;
-      e2 = ((c.sub >> ((lx >> 1) + 4u * (ly >> 1) + 16u * (lz >> 1))) & 1ull) == 0;
-      if (!e4 && !e2) {
-        if (COUNT) cnt.n_vox++;
-        const uint64_t m = s.masks[(uint64_t)c.slot * 8u + lz];
-        if ((m >> (lx + 8u * ly)) & 1ull) {
-          material = s.voxels[(uint64_t)c.slot * 512u + lx + 8u * ly + 64u * lz];
-          return kWalkHit;
-        }
+    // bit of the cell's 2^3 block in sub ((x&7)>>1) + 4 ((y&7)>>1) + 16
+    // ((z&7)>>1); its bits 1, 3 and 5 are the 4^3 block's corner bit
+    const uint32_t i2 = ((x >> 1) & 3u) | ((y << 1) & 12u) | ((z << 3) & 48u);
+    const bool e4 = ((uint32_t)(c.sub >> (i2 & 0x2Au)) & 0x00330033u) == 0;
+    const bool e2 = ((c.sub >> i2) & 1ull) == 0;
+    if (in_brick && !e4 && !e2) {
+      if (COUNT) cnt.n_vox++;
+      const uint32_t lz = z & 7u, lxy = (x & 7u) | ((y & 7u) << 3);
+      const uint64_t m = s.masks[(uint64_t)c.slot * 8u + lz];
+      if ((m >> lxy) & 1ull) {
+        material = s.voxels[(uint64_t)c.slot * 512u + lxy + 64u * lz];
+        return kWalkHit;
       }
     }
+    c.vmask = c.e128 ? 127u : (c.e32 ? 31u : 7u);
     const int32_t sh =
         c.e128 ? 7 : (c.e32 ? 5 : (!in_brick ? 3 : (e4 ? 2 : (e2 ? 1 : 0))));
-    return advance(r, s, sh, w) ? kWalkOn : kWalkOut;
+    const bool on = advance(r, s, sh, w);
+    c.dv = (x ^ (uint32_t)w.cell[0]) | (y ^ (uint32_t)w.cell[1]) | (z ^ (uint32_t)w.cell[2]);
+    return on ? kWalkOn : kWalkOut;
   }
 }
 
