@@ -127,13 +127,14 @@ __device__ __forceinline__ bool advance(const Ray &r, const DevScene &s, int32_t
   const int32_t pax = a0 ? pl[0] : (a1 ? pl[1] : pl[2]);
   const int32_t na = pax - (int32_t)((r.negmask >> ax) & 1u);
   const int32_t face = 4 - 2 * ax + (int32_t)((r.posmask >> ax) & 1u);  // entry_face
-  // compared with each axis' bounds (scalar operands) and the chosen axis'
-  // answer kept: selecting the bounds per lane would hold them in vector
-  // registers for the whole walk
-  const bool out0 = na < s.lo[0] || na >= s.hi[0];
-  const bool out1 = na < s.lo[1] || na >= s.hi[1];
-  const bool out2 = na < s.lo[2] || na >= s.hi[2];
-  if (a0 ? out0 : (a1 ? out1 : out2)) return false;
+  // compared with each axis' bounds (scalar operands; one unsigned compare
+  // per axis) and the chosen axis' answer kept, combined as lane masks:
+  // selecting the bounds per lane would hold them in vector registers for
+  // the whole walk
+  const bool out0 = (uint32_t)(na - s.lo[0]) >= (uint32_t)(s.hi[0] - s.lo[0]);
+  const bool out1 = (uint32_t)(na - s.lo[1]) >= (uint32_t)(s.hi[1] - s.lo[1]);
+  const bool out2 = (uint32_t)(na - s.lo[2]) >= (uint32_t)(s.hi[2] - s.lo[2]);
+  if ((a0 & out0) | (a1 & out1) | (!a0 & !a1 & out2)) return false;
   if constexpr (!LEVELS) {  // voxel steps only (dense grid): the other cells stay
 #pragma unroll
     for (int b = 0; b < 3; b++) w.cell[b] = b == ax ? na : w.cell[b];
@@ -307,8 +308,10 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkSt
     const bool e2 = ((c.sub >> i2) & 1ull) == 0;
     if (in_brick && !e4 && !e2) {
       if (COUNT) cnt.n_vox++;
+      // 32-bit word index: a slot holds a 1 KiB payload, so slots stay below
+      // 2^29 on any device and slot * 8 + 7 below 2^32
       const uint32_t lz = z & 7u, lxy = (x & 7u) | ((y & 7u) << 3);
-      const uint64_t m = s.masks[(uint64_t)c.slot * 8u + lz];
+      const uint64_t m = s.masks[c.slot * 8u + lz];
       if ((m >> lxy) & 1ull) {
         material = s.voxels[(uint64_t)c.slot * 512u + lxy + 64u * lz];
         return kWalkHit;
