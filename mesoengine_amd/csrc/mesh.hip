@@ -173,7 +173,9 @@ struct MeshLds {
 // 32x32 bit transpose across the 32 lanes of a half wave: lane y holds row y
 // (bit x = element [y][x]) and gets back column y (bit x = element [x][y]).
 // Stage j swaps the high-j bits of lane y (y & j == 0) with the low-j bits of
-// lane y + j.
+// lane y + j.  Both halves' results are formed and one is kept by a bit
+// select: written as a conditional, the compiler branched on the lane's bit
+// (both sides ran anyway, plus the exec-mask handling).
 __device__ __forceinline__ uint32_t transpose32(uint32_t w, uint32_t y) {
   constexpr uint32_t hi[5] = {0xFFFF0000u, 0xFF00FF00u, 0xF0F0F0F0u, 0xCCCCCCCCu, 0xAAAAAAAAu};
 #pragma unroll
@@ -181,7 +183,10 @@ __device__ __forceinline__ uint32_t transpose32(uint32_t w, uint32_t y) {
     const int j = 16 >> k;
     const uint32_t p = (uint32_t)__shfl_xor((int)w, j, 32);
     const uint32_t h = hi[k];
-    w = (y & (uint32_t)j) ? ((w & h) | ((p >> j) & ~h)) : ((w & ~h) | ((p << j) & h));
+    const uint32_t up = (w & h) | ((p >> j) & ~h);
+    const uint32_t dn = (w & ~h) | ((p << j) & h);
+    const uint32_t m = 0u - ((y >> (4 - k)) & 1u);  // all ones where y & j
+    w = (up & m) | (dn & ~m);
   }
   return w;
 }
@@ -312,6 +317,15 @@ __device__ __forceinline__ uint32_t face_row(const MeshLds &L, const PlaneView &
   return L.M[p.self + v * p.stride] & ~(p.nb >= 0 ? L.M[p.nb + v * p.stride] : 0u);
 }
 
+// face_row at word offset `off` (= v * stride) of the plane's rows, without
+// a branch: a plane on the chunk's side reads its own row as the neighbour
+// and masks it off.
+__device__ __forceinline__ uint32_t face_row_at(const MeshLds &L, const PlaneView &p, int off) {
+  const uint32_t nbm = p.nb >= 0 ? ~0u : 0u;
+  const int nb = p.nb >= 0 ? p.nb : p.self;
+  return L.M[p.self + off] & ~(L.M[nb + off] & nbm);
+}
+
 __device__ __forceinline__ uint32_t pack_quad(int f, int s, int u, int v, int w, int h) {
   int x, y, z;
   if (f < 2) {
@@ -366,7 +380,7 @@ __device__ __forceinline__ uint32_t plane_count(const MeshLds &L, int t, uint32_
   uint32_t open = 0, starts = 0, ev_prev = 0;
 #pragma unroll 4
   for (int v = 0; v < 32; v++) {
-    const uint32_t F = face_row(L, p, v);
+    const uint32_t F = face_row_at(L, p, v * p.stride);
     const uint32_t eh = L.M[p.eu + v * p.stride];
     if (v > 0) {
       const uint32_t dead = segment_fill(open & ~(F & ev_prev), open & ~starts) & open;
@@ -394,73 +408,114 @@ __device__ __forceinline__ void record_starts(StartRows &r, uint32_t st, int v) 
   for (int k = 0; k < 5; k++) r.row[k] = (r.row[k] & ~st) | (((v >> k) & 1) ? st : 0u);
 }
 
-__device__ __forceinline__ int start_row(const StartRows &r, int u) {
-  int v = 0;
+__device__ __forceinline__ uint32_t start_row(const StartRows &r, uint32_t u) {
+  uint32_t v = __builtin_amdgcn_ubfe(r.row[0], u, 1u);
 #pragma unroll
-  for (int k = 0; k < 5; k++) v |= (int)((r.row[k] >> u) & 1u) << k;
+  for (int k = 1; k < 5; k++) v |= __builtin_amdgcn_ubfe(r.row[k], u, 1u) << k;
   return v;
 }
 
+// pack_quad with plane (f, s)'s axes resolved once:
+// quad = fixed | u << su | v << sv | (w - 1) << 15 | (h - 1) << 20.
+struct PlanePack {
+  uint32_t fixed, su, sv;
+};
+
+__device__ __forceinline__ PlanePack plane_pack(int f, int s) {
+  PlanePack p;
+  if (f < 2) {  // (x, y, z) = (u, v, s)
+    p.su = 0;
+    p.sv = 5;
+    p.fixed = (uint32_t)s << 10;
+  } else if (f < 4) {  // (u, s, v)
+    p.su = 0;
+    p.sv = 10;
+    p.fixed = (uint32_t)s << 5;
+  } else {  // (s, u, v)
+    p.su = 5;
+    p.sv = 10;
+    p.fixed = (uint32_t)s;
+  }
+  p.fixed |= (uint32_t)f << 25;
+  return p;
+}
+
+// The quad of the segment that starts at u (its columns: u and the `cont`
+// bits above it), first row v, last row vend - 1.  (cont >> u) >> 1 is 0 at
+// u = 31, where the width is 1.
+__device__ __forceinline__ uint32_t closed_quad(const PlanePack &pk, uint32_t cont, uint32_t u,
+                                                uint32_t v, uint32_t vend) {
+  const uint32_t wm1 = (uint32_t)__builtin_ctz(~((cont >> u) >> 1));
+  return pk.fixed | (u << pk.su) | (v << pk.sv) | (wm1 << 15) | ((vend - v - 1u) << 20);
+}
+
 // Close the quads whose start bits are in `bits` (their last row is
-// vend - 1), in ascending u, at out[base + count ...].
-__device__ __forceinline__ void close_quads(const StartRows &r, int f, int s, uint32_t bits,
-                                            uint32_t open, uint32_t starts, int vend, uint32_t *out,
-                                            uint64_t base, uint64_t cap, uint32_t &count) {
+// vend - 1), in ascending u, at out[base + count ...]; CHECK: stop writing at
+// cap (a caller that knows the plane fits passes false and skips the tests).
+template <bool CHECK>
+__device__ __forceinline__ void close_quads(const StartRows &r, const PlanePack &pk, uint32_t bits,
+                                            uint32_t open, uint32_t starts, uint32_t vend,
+                                            uint32_t *out, uint64_t base, uint64_t cap,
+                                            uint32_t &count) {
   const uint32_t cont = open & ~starts;
   while (bits) {  // two quads per trip
-    const int u1 = __ffs(bits) - 1;
+    const uint32_t u1 = (uint32_t)__builtin_ctz(bits);
     bits &= bits - 1;
     const bool two = bits != 0u;
-    const int u2 = two ? __ffs(bits) - 1 : u1;
+    const uint32_t u2 = two ? (uint32_t)__builtin_ctz(bits) : u1;
     bits &= two ? bits - 1 : bits;
-    const uint32_t a1 = u1 < 31 ? ~(cont >> (u1 + 1)) : 0u;  // zero where the segment ends
-    const uint32_t a2 = u2 < 31 ? ~(cont >> (u2 + 1)) : 0u;
-    const int w1 = 1 + (u1 < 31 ? (__ffs(a1) - 1) : 0);
-    const int w2 = 1 + (u2 < 31 ? (__ffs(a2) - 1) : 0);
-    const int v1 = start_row(r, u1), v2 = start_row(r, u2);
+    const uint32_t q1 = closed_quad(pk, cont, u1, start_row(r, u1), vend);
+    const uint32_t q2 = closed_quad(pk, cont, u2, start_row(r, u2), vend);
     const uint64_t k = base + count;
     count += two ? 2u : 1u;
-    if (k < cap) out[k] = pack_quad(f, s, u1, v1, w1, vend - v1);
-    if (two && k + 1 < cap) out[k + 1] = pack_quad(f, s, u2, v2, w2, vend - v2);
+    if (!CHECK || k < cap) out[k] = q1;
+    if (two && (!CHECK || k + 1 < cap)) out[k + 1] = q2;
   }
 }
 
-// Step 2b: write the quads of plane t at out[base ...] in canonical order:
-// by the row where they end, then by start u (the order the sweep closes
-// them in; for 1x1 culled faces, row then u).
-template <bool GREEDY>
-__device__ __forceinline__ void plane_emit(MeshLds &L, int t, uint32_t *out, uint64_t base,
-                                           uint64_t cap) {
+// Step 2b, culled: the faces of plane t at out[base ...], by row, then u.
+__device__ __forceinline__ void plane_emit_culled(MeshLds &L, int t, uint32_t *out, uint64_t base,
+                                                  uint64_t cap) {
   const int f = t >> 5, s = t & 31;
   const PlaneView p = plane_view(f, s);
   uint32_t count = 0;
-  if (!GREEDY) {
-    for (int v = 0; v < 32; v++) {
-      uint32_t row = face_row(L, p, v);
-      // two faces per trip
-      while (row) {
-        const int u1 = __ffs(row) - 1;
-        row &= row - 1;
-        const bool two = row != 0u;
-        const int u2 = two ? __ffs(row) - 1 : u1;
-        row &= two ? row - 1 : row;
-        const uint64_t k = base + count;
-        count += two ? 2u : 1u;
-        if (k < cap) out[k] = pack_quad(f, s, u1, v, 1, 1);
-        if (two && k + 1 < cap) out[k + 1] = pack_quad(f, s, u2, v, 1, 1);
-      }
+  for (int v = 0; v < 32; v++) {
+    uint32_t row = face_row(L, p, v);
+    // two faces per trip
+    while (row) {
+      const int u1 = __ffs(row) - 1;
+      row &= row - 1;
+      const bool two = row != 0u;
+      const int u2 = two ? __ffs(row) - 1 : u1;
+      row &= two ? row - 1 : row;
+      const uint64_t k = base + count;
+      count += two ? 2u : 1u;
+      if (k < cap) out[k] = pack_quad(f, s, u1, v, 1, 1);
+      if (two && k + 1 < cap) out[k + 1] = pack_quad(f, s, u2, v, 1, 1);
     }
-    return;
   }
+}
+
+// Step 2b, greedy: write the quads of plane t at out[base ...] in canonical
+// order: by the row where they end, then by start u (the order the sweep
+// closes them in).
+template <bool CHECK>
+__device__ __forceinline__ void plane_emit_greedy(MeshLds &L, int t, uint32_t *out, uint64_t base,
+                                                  uint64_t cap) {
+  const int f = t >> 5, s = t & 31;
+  const PlaneView p = plane_view(f, s);
+  const PlanePack pk = plane_pack(f, s);
+  uint32_t count = 0;
   uint32_t open = 0, starts = 0, ev_prev = 0;
   StartRows r = {{0u, 0u, 0u, 0u, 0u}};
-  for (int v = 0; v < 32; v++) {
-    const uint32_t F = face_row(L, p, v);
-    const uint32_t eh = L.M[p.eu + v * p.stride];
-    const uint32_t ev = L.M[p.ev + v * p.stride];
+  int off = 0;  // v * stride, kept as a running sum (no multiply per row)
+  for (int v = 0; v < 32; v++, off += p.stride) {
+    const uint32_t F = face_row_at(L, p, off);
+    const uint32_t eh = L.M[p.eu + off];
+    const uint32_t ev = L.M[p.ev + off];
     if (v > 0) {
       const uint32_t dead = segment_fill(open & ~(F & ev_prev), open & ~starts) & open;
-      close_quads(r, f, s, starts & dead, open, starts, v, out, base, cap, count);
+      close_quads<CHECK>(r, pk, starts & dead, open, starts, (uint32_t)v, out, base, cap, count);
       open &= ~dead;
       starts &= ~dead;
     }
@@ -471,7 +526,21 @@ __device__ __forceinline__ void plane_emit(MeshLds &L, int t, uint32_t *out, uin
     record_starts(r, st, v);
     ev_prev = ev;
   }
-  close_quads(r, f, s, starts, open, starts, 32, out, base, cap, count);
+  close_quads<CHECK>(r, pk, starts, open, starts, 32u, out, base, cap, count);
+}
+
+// Step 2b: the quads of plane t (`total` of them, from plane_count) at
+// out[base ...], written unchecked when they all fit below cap.
+template <bool GREEDY>
+__device__ __forceinline__ void plane_emit(MeshLds &L, int t, uint32_t *out, uint64_t base,
+                                           uint64_t cap, uint32_t total) {
+  if (!GREEDY) {
+    plane_emit_culled(L, t, out, base, cap);
+  } else if (base + total <= cap) {
+    plane_emit_greedy<false>(L, t, out, base, cap);
+  } else {
+    plane_emit_greedy<true>(L, t, out, base, cap);
+  }
 }
 
 // position of the r-th (0-based) set bit of x; x has more than r set bits
@@ -669,7 +738,7 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_batched_kernel(
     __syncthreads();
     const uint64_t base = ((uint64_t)L.base_hi << 32) | L.base_lo;
     if constexpr (GREEDY) {
-      if (t < kPlanes) plane_emit<true>(L, t, quads, base + excl, capacity);
+      if (t < kPlanes) plane_emit<true>(L, t, quads, base + excl, capacity, mine);
     } else {
       culled_emit_balanced(L, chunk_total, quads, base, capacity);
     }
@@ -708,7 +777,7 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_slotted_kernel(
     const uint64_t base = (uint64_t)slot * per_slot, cap = base + per_slot;
     if (t == 0) slot_count[slot] = chunk_total;
     if constexpr (GREEDY) {
-      if (t < kPlanes) plane_emit<true>(L, t, quads, base + excl, cap);
+      if (t < kPlanes) plane_emit<true>(L, t, quads, base + excl, cap, mine);
     } else {
       if (t < kPlanes) {
         L.pex[t] = excl;
@@ -738,7 +807,7 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_plain_kernel(const void *voxe
   const uint32_t chunk_total = block_exclusive_scan(L, mine, excl);
   if (t == 0) *count = chunk_total;
   if constexpr (GREEDY) {
-    if (t < kPlanes) plane_emit<true>(L, t, quads, excl, capacity);
+    if (t < kPlanes) plane_emit<true>(L, t, quads, excl, capacity, mine);
   } else {
     if (t < kPlanes) {
       L.pex[t] = excl;
