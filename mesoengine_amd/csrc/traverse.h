@@ -308,8 +308,8 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkSt
     const bool e2 = ((c.sub >> i2) & 1ull) == 0;
     if (in_brick && !e4 && !e2) {
       if (COUNT) cnt.n_vox++;
-      // 32-bit word index: a slot holds a 1 KiB payload, so slots stay below
-      // 2^29 on any device and slot * 8 + 7 below 2^32
+      // 32-bit word index: a scene holds fewer than 2^29 bricks (checked
+      // when it is generated or uploaded), so slot * 8 + 7 is below 2^32
       const uint32_t lz = z & 7u, lxy = (x & 7u) | ((y & 7u) << 3);
       const uint64_t m = s.masks[c.slot * 8u + lz];
       if ((m >> lxy) & 1ull) {
