@@ -27,9 +27,13 @@ __device__ __forceinline__ int entry_face(int axis, int step) {
 __device__ __forceinline__ int face_axis(int f) { return f < 2 ? 2 : (f < 4 ? 1 : 0); }
 
 struct Ray {
-  float o[3], d[3], inv[3];
+  float o[3], d[3], inv[3];  // inv: +inf where d = 0
   int32_t step[3];
-  int32_t pos[3];    // 1 where step > 0: the exit plane of block b is b + 1
+  // the exit plane of block b is b + pos: 1 where step > 0, 0 where step < 0,
+  // and 2 where step = 0, so that plane lies beyond the origin and its t,
+  // (plane - o) * inf, is +inf: an axis the ray does not move along is never
+  // the nearest, with no select per step
+  int32_t pos[3];
   uint32_t posmask;  // bit a: step[a] > 0
   uint32_t negmask;  // bit a: step[a] < 0 (the next cell is one below the plane)
 };
@@ -46,8 +50,8 @@ __device__ __forceinline__ void ray_setup(Ray &r, const float o[3], const float 
     const float inv = 1.0f / d[a];
     const bool pos = d[a] > 0.0f, neg = d[a] < 0.0f;
     r.step[a] = pos ? 1 : (neg ? -1 : 0);
-    r.inv[a] = (pos || neg) ? inv : 0.0f;
-    r.pos[a] = r.step[a] > 0 ? 1 : 0;
+    r.inv[a] = (pos || neg) ? inv : __builtin_inff();
+    r.pos[a] = pos ? 1 : (neg ? 0 : 2);
     r.posmask |= (r.step[a] > 0 ? 1u : 0u) << a;
     r.negmask |= (r.step[a] < 0 ? 1u : 0u) << a;
   }
@@ -116,8 +120,10 @@ __device__ __forceinline__ bool advance(const Ray &r, const DevScene &s, int32_t
   for (int a = 0; a < 3; a++) {
     bc[a] = w.cell[a] >> sh;
     pl[a] = (bc[a] + r.pos[a]) << sh;  // the exit plane on this axis
-    const float t = plane_t(r.o[a], r.inv[a], (float)pl[a]);
-    tb[a] = r.step[a] != 0 ? t : __builtin_inff();
+    // +inf where step = 0: pl is at least the cell + 2 there, and the
+    // origin's coordinate stays below the cell + 2 (the cell is floor(o),
+    // or for a secondary ray the hit cell, whose face holds o)
+    tb[a] = plane_t(r.o[a], r.inv[a], (float)pl[a]);
   }
   // the chosen axis by selects and bit extracts only: an index into the
   // ray's arrays would move them out of registers

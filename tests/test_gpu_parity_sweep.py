@@ -85,3 +85,61 @@ def test_random_cameras_match_oracle(ctx, kind, seed):
         hits += int((idx >= 0).sum())
     assert hits > 1000, "the sweep saw too few hits to test anything"
     sc.destroy()
+
+
+def _axis_cameras(n):
+    """Cameras looking straight along each axis, with up along another axis:
+    a W x 1 image has primary rays with one component exactly 0, a 1 x H
+    image another."""
+    c = n * 0.5 + 0.25
+    out = []
+    for a in range(3):
+        for sgn in (1.0, -1.0):
+            eye = [c, c, c]
+            eye[a] -= sgn * 0.45 * n
+            target = [c, c, c]
+            up = [0.0, 0.0, 0.0]
+            up[(a + 1) % 3] = 1.0
+            out.append((eye, target, up))
+    return out
+
+
+@pytest.mark.parametrize("kind", ["dense", "height", "two"])
+def test_axis_aligned_rays_match_oracle(ctx, kind):
+    """Rays with direction components exactly 0 (primary rays of axis-aligned
+    cameras in W x 1 and 1 x H images, shadow rays toward an axis-aligned
+    light) against the oracle, with and without the step counters."""
+    sc, osc, n = _scene(ctx, kind)
+    flags = O.RAY_SHADOW | O.RAY_AO
+    zero_primary = 0
+    hits = 0
+    for i, (eye, target, up) in enumerate(_axis_cameras(n)):
+        for w, h in ((64, 1), (1, 48), (24, 18)):
+            light = (0.0, 1.0, 0.0) if i % 2 == 0 else (1.0, 0.0, 0.0)
+            cam = R.camera(eye, target, up, 60.0, w, h)
+            ocam = O.camera(eye, target, up, 60.0, w, h)
+            for py in range(h):
+                for px in range(w):
+                    zero_primary += int((O.primary_ray(ocam, px, py)[1] == 0.0).any())
+            ref = O.raymarch(osc, ocam, flags, O.shading(light, 7 + i, 4, 12.0), threads=8)
+            for count in (True, False):
+                tg = R.RayTargets(ctx, w, h, counters=count)
+                if count:
+                    tg.counters.zero()
+                R.raymarch(ctx, sc, cam, flags | (L.RAY_COUNT if count else 0),
+                           R.shading(light, 7 + i, 4, 12.0), tg)
+                ctx.sync()
+                idx, face, t, shade = tg.results()
+                what = "%s camera %d %dx%d count %d" % (kind, i, w, h, count)
+                assert np.array_equal(idx, ref[0]), what
+                assert np.array_equal(face, ref[1]), what
+                assert np.array_equal(t.view(np.uint32), ref[2].view(np.uint32)), what
+                assert np.array_equal(shade.view(np.uint32), ref[3].view(np.uint32)), what
+                if count:
+                    got = tg.counts()
+                    for k, v in ref[4].as_dict().items():
+                        assert got[k] == v, (what, k)
+            hits += int((ref[0] >= 0).sum())
+    assert zero_primary > 100, "no primary ray had a zero component"
+    assert hits > 200, "too few hits to test anything"
+    sc.destroy()
