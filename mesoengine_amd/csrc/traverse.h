@@ -119,7 +119,9 @@ __device__ __forceinline__ int32_t low_mask(int32_t k) {
 // itself, so the clamp keeps them, which is the plain voxel step.  Written
 // without branches (selects only), so that the lanes of a wave stepping at
 // different levels stay converged.  Returns false when the step along the
-// chosen axis leaves the scene's solid box (nothing solid lies beyond it).
+// chosen axis leaves the scene's solid box (nothing solid lies beyond it);
+// the step is computed all the same (no branch), and a walk that gets false
+// ends without reading the cell.
 template <bool LEVELS = true>
 __device__ __forceinline__ bool advance(const Ray &r, const DevScene &s, int32_t sh, WalkState &w) {
   const int32_t n = s.n;
@@ -149,13 +151,13 @@ __device__ __forceinline__ bool advance(const Ray &r, const DevScene &s, int32_t
   const bool out0 = (uint32_t)(na - s.lo[0]) >= (uint32_t)(s.hi[0] - s.lo[0]);
   const bool out1 = (uint32_t)(na - s.lo[1]) >= (uint32_t)(s.hi[1] - s.lo[1]);
   const bool out2 = (uint32_t)(na - s.lo[2]) >= (uint32_t)(s.hi[2] - s.lo[2]);
-  if ((a0 & out0) | (a1 & out1) | (!a0 & !a1 & out2)) return false;
+  const bool out = (a0 & out0) | (a1 & out1) | (!a0 & !a1 & out2);
   if constexpr (!LEVELS) {  // voxel steps only (dense grid): the other cells stay
 #pragma unroll
     for (int b = 0; b < 3; b++) w.cell[b] = b == ax ? na : w.cell[b];
     w.t = t;
     w.face = face;
-    return true;
+    return !out;
   }
   const int32_t span = low_mask(sh);
 #pragma unroll
@@ -176,7 +178,7 @@ __device__ __forceinline__ bool advance(const Ray &r, const DevScene &s, int32_t
   }
   w.t = t;
   w.face = face;
-  return true;
+  return !out;
 }
 
 // The 4x4 2^2 blocks of an 8x8 plane of a brick (bit x + 8 y) that hold a set
@@ -246,9 +248,11 @@ struct WalkCache {
 
 enum : int { kWalkOn = 0, kWalkHit = 1, kWalkOut = 2 };
 
-// One step of a walk: kWalkHit when the current cell is solid (material set),
-// kWalkOut when the entry t has reached tmax or the ray leaves the scene,
-// else kWalkOn after stepping to the next cell.  KIND: 1 dense u8 (voxel
+// One step of a walk from a cell whose entry t is below tmax: kWalkHit when
+// the cell is solid (material set), kWalkOut when the ray leaves the scene or
+// the next cell's entry t reaches tmax, else kWalkOn after stepping to the
+// next cell.  (Testing the next cell's t here, not at the top of the next
+// step, saves a branch per step; walk tests the first cell's.)  KIND: 1 dense u8 (voxel
 // steps only), 2 brick map, 3 two-level.  A brick scene steps at the coarsest
 // empty level holding the cell: 128^3 block, 32^3 block (its word of
 // allocated bricks, which also names the empty bricks without reading their
@@ -259,7 +263,6 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkSt
                                          WalkCache &c, float tmax, uint16_t &material,
                                          StepCounts &cnt) {
   const int32_t n = s.n;
-  if (!(w.t < tmax)) return kWalkOut;
   if (COUNT) cnt.n_iter++;
   if constexpr (KIND == 1) {
     if (COUNT) cnt.n_vox++;
@@ -268,7 +271,7 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkSt
       material = v;
       return kWalkHit;
     }
-    return advance<false>(r, s, 0, w) ? kWalkOn : kWalkOut;
+    return advance<false>(r, s, 0, w) && w.t < tmax ? kWalkOn : kWalkOut;
   } else {
     const uint32_t n128 = (uint32_t)(n + 127) >> 7, n32 = (uint32_t)(n + 31) >> 5;
     // one flat pass over the levels (no nesting, so a wave whose lanes sit
@@ -335,7 +338,7 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkSt
     c.vmask = c.e128 ? 127u : (c.e32 ? 31u : 7u);
     const int32_t sh =
         c.e128 ? 7 : (c.e32 ? 5 : (!in_brick ? 3 : (e4 ? 2 : (e2 ? 1 : 0))));
-    const bool on = advance(r, s, sh, w);
+    const bool on = advance(r, s, sh, w) && w.t < tmax;
     c.dv = (x ^ (uint32_t)w.cell[0]) | (y ^ (uint32_t)w.cell[1]) | (z ^ (uint32_t)w.cell[2]);
     return on ? kWalkOn : kWalkOut;
   }
@@ -346,6 +349,7 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const Ray &r, WalkSt
 template <int KIND, bool COUNT>
 __device__ __forceinline__ bool walk(const DevScene &s, const Ray &r, WalkState &w, float tmax,
                                      uint16_t &material, StepCounts &cnt) {
+  if (!(w.t < tmax)) return false;
   WalkCache c;
   c.reset();
   for (;;) {
