@@ -36,6 +36,9 @@ struct Ray {
   int32_t pos[3];
   uint32_t posmask;  // bit a: step[a] > 0
   uint32_t negmask;  // bit a: step[a] < 0 (the next cell is one below the plane)
+  // the direction the cell can move in on axis a: step[a] > 0, or on an
+  // axis the ray does not move along, floor(o[a]) above the start cell
+  bool up[3];
 };
 
 __device__ __forceinline__ void ray_setup(Ray &r, const float o[3], const float d[3]) {
@@ -52,6 +55,7 @@ __device__ __forceinline__ void ray_setup(Ray &r, const float o[3], const float 
     r.step[a] = pos ? 1 : (neg ? -1 : 0);
     r.inv[a] = (pos || neg) ? inv : __builtin_inff();
     r.pos[a] = pos ? 1 : (neg ? 0 : 2);
+    r.up[a] = pos;
     r.posmask |= (r.step[a] > 0 ? 1u : 0u) << a;
     r.negmask |= (r.step[a] < 0 ? 1u : 0u) << a;
   }
@@ -167,13 +171,14 @@ __device__ __forceinline__ bool advance(const Ray &r, const DevScene &s, int32_t
     int32_t c = floor_i32(p);
     // clamp into the block [lo, hi], then not below the cell when stepping
     // up or above it when stepping down: the cell lies in [lo, hi], so that
-    // is one clamp into [cell or lo, cell or hi] (a single med3)
+    // is one clamp between the cell and one bound (a single med3).  On an
+    // axis the ray does not move along, c is floor(o) on every step and
+    // the cell only moves toward it, so the side up[] fixed at the start
+    // holds for the whole walk and the same clamp gives clamp(c, lo, hi).
     const int32_t lo = bc[b] << sh;
     int32_t hi = lo + span;
     hi = hi > n - 1 ? n - 1 : hi;
-    const int32_t cl = r.step[b] > 0 ? w.cell[b] : lo;
-    const int32_t ch = r.step[b] < 0 ? w.cell[b] : hi;
-    c = med3_i32(c, cl, ch);
+    c = med3_i32(c, w.cell[b], r.up[b] ? hi : lo);
     w.cell[b] = b == ax ? na : c;
   }
   w.t = t;
@@ -481,6 +486,9 @@ __device__ __forceinline__ void occluded_begin(const float o[3], const float d[3
   w.cell[2] = start[2];
   w.t = 0.0f;
   w.face = kFaceInside;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+    r.up[a] = r.step[a] > 0 || (r.step[a] == 0 && (int32_t)__builtin_floorf(o[a]) > start[a]);
 }
 
 template <int KIND, bool COUNT>
