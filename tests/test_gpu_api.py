@@ -45,6 +45,33 @@ def test_dense_grid_u8_chunks(ctx, mode):
         assert np.array_equal(quads[prefix[c]:prefix[c] + counts[c]], refs[c]), c
 
 
+@pytest.mark.parametrize("mode", [L.MESH_CULLED, L.MESH_GREEDY])
+def test_batched_mesh_past_capacity_keeps_the_prefix(ctx, mode):
+    """A quad buffer smaller than the batch's quads: the counts, prefixes and
+    total are those of the whole batch, and the buffer holds the first
+    `capacity` quads of the whole output (planes that straddle the end are
+    written up to it, later ones not at all)."""
+    n = 64
+    vox = ctx.buffer(2 * 32768 * n)
+    L.check(ctx.lib.meso_generate_chunks(ctx.h, 31, n, vox.ptr, None))
+    src = R.chunk_source(vox, 2, n)
+    full = R.MeshTargets(ctx, n, 4 << 20)
+    R.mesh_chunks(ctx, src, mode, full)
+    ctx.sync()
+    fc, fp, ft, fq = full.results()
+    assert 1000 < ft <= 4 << 20
+    ref_vox, _ = O.gen_chunks(31, n)
+    for c in range(0, n, 9):
+        assert np.array_equal(fq[fp[c]:fp[c] + fc[c]], O.mesh_chunk(ref_vox[c], mode == L.MESH_GREEDY))
+    for cap in (ft // 3 + 7, ft - 1):
+        mt = R.MeshTargets(ctx, n, cap)
+        R.mesh_chunks(ctx, src, mode, mt)
+        ctx.sync()
+        c, p, t, q = mt.results()
+        assert t == ft and np.array_equal(c, fc) and np.array_equal(p, fp), cap
+        assert np.array_equal(q, fq[:cap]), cap
+
+
 def test_graph_replay_matches_eager(ctx):
     n = 128
     vox = ctx.buffer(2 * 32768 * n)
