@@ -473,29 +473,6 @@ __device__ __forceinline__ void close_quads(const StartRows &r, const PlanePack 
   }
 }
 
-// Step 2b, culled: the faces of plane t at out[base ...], by row, then u.
-__device__ __forceinline__ void plane_emit_culled(MeshLds &L, int t, uint32_t *out, uint64_t base,
-                                                  uint64_t cap) {
-  const int f = t >> 5, s = t & 31;
-  const PlaneView p = plane_view(f, s);
-  uint32_t count = 0;
-  for (int v = 0; v < 32; v++) {
-    uint32_t row = face_row(L, p, v);
-    // two faces per trip
-    while (row) {
-      const int u1 = __ffs(row) - 1;
-      row &= row - 1;
-      const bool two = row != 0u;
-      const int u2 = two ? __ffs(row) - 1 : u1;
-      row &= two ? row - 1 : row;
-      const uint64_t k = base + count;
-      count += two ? 2u : 1u;
-      if (k < cap) out[k] = pack_quad(f, s, u1, v, 1, 1);
-      if (two && k + 1 < cap) out[k + 1] = pack_quad(f, s, u2, v, 1, 1);
-    }
-  }
-}
-
 // Step 2b, greedy: write the quads of plane t at out[base ...] in canonical
 // order: by the row where they end, then by start u (the order the sweep
 // closes them in).
@@ -529,18 +506,16 @@ __device__ __forceinline__ void plane_emit_greedy(MeshLds &L, int t, uint32_t *o
   close_quads<CHECK>(r, pk, starts, open, starts, 32u, out, base, cap, count);
 }
 
-// Step 2b: the quads of plane t (`total` of them, from plane_count) at
-// out[base ...], written unchecked when they all fit below cap.
-template <bool GREEDY>
+// Step 2b, greedy: the quads of plane t (`total` of them, from plane_count)
+// at out[base ...], written unchecked when they all fit below cap.  (Culled
+// meshing shares its faces out over the whole workgroup instead:
+// culled_emit_balanced.)
 __device__ __forceinline__ void plane_emit(MeshLds &L, int t, uint32_t *out, uint64_t base,
                                            uint64_t cap, uint32_t total) {
-  if (!GREEDY) {
-    plane_emit_culled(L, t, out, base, cap);
-  } else if (base + total <= cap) {
+  if (base + total <= cap)
     plane_emit_greedy<false>(L, t, out, base, cap);
-  } else {
+  else
     plane_emit_greedy<true>(L, t, out, base, cap);
-  }
 }
 
 // position of the r-th (0-based) set bit of x; x has more than r set bits
@@ -738,7 +713,7 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_batched_kernel(
     __syncthreads();
     const uint64_t base = ((uint64_t)L.base_hi << 32) | L.base_lo;
     if constexpr (GREEDY) {
-      if (t < kPlanes) plane_emit<true>(L, t, quads, base + excl, capacity, mine);
+      if (t < kPlanes) plane_emit(L, t, quads, base + excl, capacity, mine);
     } else {
       culled_emit_balanced(L, chunk_total, quads, base, capacity);
     }
@@ -777,7 +752,7 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_slotted_kernel(
     const uint64_t base = (uint64_t)slot * per_slot, cap = base + per_slot;
     if (t == 0) slot_count[slot] = chunk_total;
     if constexpr (GREEDY) {
-      if (t < kPlanes) plane_emit<true>(L, t, quads, base + excl, cap, mine);
+      if (t < kPlanes) plane_emit(L, t, quads, base + excl, cap, mine);
     } else {
       if (t < kPlanes) {
         L.pex[t] = excl;
@@ -807,7 +782,7 @@ __global__ void __launch_bounds__(kMeshBlock) mesh_plain_kernel(const void *voxe
   const uint32_t chunk_total = block_exclusive_scan(L, mine, excl);
   if (t == 0) *count = chunk_total;
   if constexpr (GREEDY) {
-    if (t < kPlanes) plane_emit<true>(L, t, quads, excl, capacity, mine);
+    if (t < kPlanes) plane_emit(L, t, quads, excl, capacity, mine);
   } else {
     if (t < kPlanes) {
       L.pex[t] = excl;
