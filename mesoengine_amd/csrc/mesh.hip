@@ -91,6 +91,47 @@ __device__ __forceinline__ Vox8<ES> load8(const ChunkAddr &a, int x0, int y, int
   return r;
 }
 
+// Per 16-bit half of a: min(half, 1), i.e. 1 where the half is nonzero (one
+// v_pk_min_u16; written as a generic vector min, the compiler turns it into a
+// compare and select per half).
+__device__ __forceinline__ uint32_t halves_nonzero(uint32_t a) {
+  uint32_t r;
+  asm("v_pk_min_u16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(0x00010001u));
+  return r;
+}
+
+// Bit j set where voxel j of four u16 words (voxel 2i in the low half of
+// a[i], 2i + 1 in the high half) is nonzero.  halves_nonzero leaves a 1 in
+// bit 0 and bit 16 of each word; the four words are shifted together so the
+// low halves land on bits 0, 2, 4, 6 and the high halves on bits 16, 18, 20,
+// 22, and a shift by 15 takes the latter to bits 1, 3, 5, 7.
+__device__ __forceinline__ uint32_t nonzero8(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3) {
+  const uint32_t g = halves_nonzero(a0) | (halves_nonzero(a1) << 2) |
+                     (halves_nonzero(a2) << 4) | (halves_nonzero(a3) << 6);
+  return (g | (g >> 15)) & 0xFFu;
+}
+
+// The four mask pieces of 8 u16 voxels (words w) at x = 8k .. 8k + 7:
+// occupancy, and equality with the next voxel along x (nx: the first word of
+// x + 8), y (wy) and z (wz).  The x pairs (v[j], v[j + 1]) are word i against
+// the funnel shift of (w[i + 1]:w[i]) by 16.
+struct Pieces {
+  uint32_t o, ex, ey, ez;
+};
+
+__device__ __forceinline__ Pieces pieces_u16(const uint32_t w[4], uint32_t nx,
+                                                const uint32_t wy[4], const uint32_t wz[4],
+                                                bool last_k, bool y_in, bool z_in) {
+  const uint32_t s0 = (w[0] >> 16) | (w[1] << 16), s1 = (w[1] >> 16) | (w[2] << 16),
+                 s2 = (w[2] >> 16) | (w[3] << 16), s3 = (w[3] >> 16) | (nx << 16);
+  Pieces p;
+  p.o = nonzero8(w[0], w[1], w[2], w[3]);
+  p.ex = ~nonzero8(w[0] ^ s0, w[1] ^ s1, w[2] ^ s2, w[3] ^ s3) & (last_k ? 0x7Fu : 0xFFu);
+  p.ey = y_in ? ~nonzero8(w[0] ^ wy[0], w[1] ^ wy[1], w[2] ^ wy[2], w[3] ^ wy[3]) & 0xFFu : 0u;
+  p.ez = z_in ? ~nonzero8(w[0] ^ wz[0], w[1] ^ wz[1], w[2] ^ wz[2], w[3] ^ wz[3]) & 0xFFu : 0u;
+  return p;
+}
+
 template <int ES>
 __device__ __forceinline__ Vox8<ES> zero8() {
   Vox8<ES> r;
@@ -225,15 +266,22 @@ __device__ __forceinline__ void build_masks(MeshLds &L, const Rows &a) {
       const int y = (int)(r & 31u), z = (int)(r >> 5);
       const uint32_t next0 = __shfl_down(c[b].w[0], 1, 64);  // first voxels of x + 8
       uint32_t o = 0, ex = 0, ey = 0, ez = 0;
+      if constexpr (ES == 2) {
+        const Pieces p = pieces_u16(c[b].w, next0, cy[b].w, cz[b].w, k == 3, y < 31, z < 31);
+        o = p.o;
+        ex = p.ex;
+        ey = p.ey;
+        ez = p.ez;
+      } else {
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const uint32_t v = c[b].get(j);
-        o |= (uint32_t)(v != 0u) << j;
-        const uint32_t vx =
-            j < 7 ? c[b].get(j + 1) : (ES == 1 ? next0 & 0xFFu : next0 & 0xFFFFu);
-        ex |= (uint32_t)(v == vx && (j < 7 || k < 3)) << j;
-        ey |= (uint32_t)(y < 31 && v == cy[b].get(j)) << j;
-        ez |= (uint32_t)(z < 31 && v == cz[b].get(j)) << j;
+        for (int j = 0; j < 8; j++) {
+          const uint32_t v = c[b].get(j);
+          o |= (uint32_t)(v != 0u) << j;
+          const uint32_t vx = j < 7 ? c[b].get(j + 1) : next0 & 0xFFu;
+          ex |= (uint32_t)(v == vx && (j < 7 || k < 3)) << j;
+          ey |= (uint32_t)(y < 31 && v == cy[b].get(j)) << j;
+          ez |= (uint32_t)(z < 31 && v == cz[b].get(j)) << j;
+        }
       }
       const int sh = 8 * k;
       o = row_or(o << sh);

@@ -251,10 +251,17 @@ def headline(ctx, steps=20, warmup=3, seed=5, log=print):
     try:
         tune = fr.tune()
         log("frame render tune: %s" % tune)
+        import ctypes as C
+        shape = L.TuneResult()
+        shape.size = C.sizeof(shape)
+        shape.variant, shape.grid = tune["variant"], tune["grid"]
+        shape.tile_w, shape.tile_h = tune["tile"]
+        if not R.matches_tiled_frame(ctx, fr.scene, fr.cam, fr.flags, fr.sh, fr.tg, shape):
+            raise RuntimeError("the tuned render shape's frame differs from the 16x16-tile frame")
+        log("frame render: the tuned shape's frame equals the 16x16-tile frame")
         cnt = fr.count()
         rays = cnt["rays"] + cnt["shadow_rays"] + cnt["ao_rays"]
         alg = frame_bytes(cnt)
-        import ctypes as C
         steps += steps % 2  # whole pairs: the timed frames end on the start occupancy
         warmup += warmup % 2
         s2 = C.c_void_p()

@@ -226,6 +226,24 @@ def raymarch(ctx, scene, cam, flags, sh, targets, stream=None):
                                 byref(targets.out), stream))
 
 
+def matches_tiled_frame(ctx, scene, cam, flags, sh, targets, shape):
+    """One frame with the launch shape `shape` (a TuneResult) pinned, against
+    one with 16x16 tiles (variant 0).  Returns True when index, face, t and
+    shade are the same bits; `shape` stays pinned afterwards."""
+    def frame(pin):
+        check(ctx.lib.meso_autotune_pin_raymarch(ctx.h, scene.h, byref(cam), flags, byref(pin)))
+        raymarch(ctx, scene, cam, flags, sh, targets)
+        ctx.sync()
+        return targets.results()
+
+    tiled = L.TuneResult()
+    tiled.size = C.sizeof(tiled)
+    tiled.variant, tiled.grid, tiled.tile_w, tiled.tile_h = 0, 0, 16, 16
+    ref = frame(tiled)
+    got = frame(shape)
+    return all(np.array_equal(a.view(np.uint8), b.view(np.uint8)) for a, b in zip(ref, got))
+
+
 def chunk_source(buf, elem_bytes, n_chunks, layout=L.CHUNKS_PACKED, grid_n=0):
     s = L.ChunkSource()
     s.size = C.sizeof(s)
