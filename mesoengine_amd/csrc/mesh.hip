@@ -127,8 +127,8 @@ __device__ __forceinline__ Pieces pieces_u16(const uint32_t w[4], uint32_t nx,
   Pieces p;
   p.o = nonzero8(w[0], w[1], w[2], w[3]);
   p.ex = ~nonzero8(w[0] ^ s0, w[1] ^ s1, w[2] ^ s2, w[3] ^ s3) & (last_k ? 0x7Fu : 0xFFu);
-  p.ey = y_in ? ~nonzero8(w[0] ^ wy[0], w[1] ^ wy[1], w[2] ^ wy[2], w[3] ^ wy[3]) & 0xFFu : 0u;
-  p.ez = z_in ? ~nonzero8(w[0] ^ wz[0], w[1] ^ wz[1], w[2] ^ wz[2], w[3] ^ wz[3]) & 0xFFu : 0u;
+  p.ey = ~nonzero8(w[0] ^ wy[0], w[1] ^ wy[1], w[2] ^ wy[2], w[3] ^ wy[3]) & (y_in ? 0xFFu : 0u);
+  p.ez = ~nonzero8(w[0] ^ wz[0], w[1] ^ wz[1], w[2] ^ wz[2], w[3] ^ wz[3]) & (z_in ? 0xFFu : 0u);
   return p;
 }
 
@@ -257,8 +257,9 @@ __device__ __forceinline__ void build_masks(MeshLds &L, const Rows &a) {
       const uint32_t r = (wave + (it + b) * nwaves) * 16u + (lane >> 2);
       const int y = (int)(r & 31u), z = (int)(r >> 5);
       c[b] = load8<ES>(a, 8 * k, y, z);
-      cy[b] = y < 31 ? load8<ES>(a, 8 * k, y + 1, z) : zero8<ES>();
-      cz[b] = z < 31 ? load8<ES>(a, 8 * k, y, z + 1) : zero8<ES>();
+      // the row itself stands in past the chunk's edge (masked off below)
+      cy[b] = load8<ES>(a, 8 * k, y < 31 ? y + 1 : y, z);
+      cz[b] = load8<ES>(a, 8 * k, y, z < 31 ? z + 1 : z);
     }
 #pragma unroll
     for (uint32_t b = 0; b < kPer; b++) {
