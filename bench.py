@@ -119,6 +119,7 @@ def raymarch_leg(ctx, R, L, W, steps, warmup):
         best[1], best[2], best[0]))
     if not R.matches_tiled_frame(ctx, sc, cam, flags, sh, tg, pin):
         raise SystemExit("bench: the pinned shape's frame differs from the 16x16-tile frame")
+    log("raymarch: the pinned shape's frame equals the 16x16-tile frame")
     # untimed counting pass: the algorithmic byte model needs the step counts
     R.raymarch(ctx, sc, cam, flags | L.RAY_COUNT, sh, tg)
     ctx.sync()
