@@ -72,6 +72,63 @@ def test_batched_mesh_past_capacity_keeps_the_prefix(ctx, mode):
         assert np.array_equal(q, fq[:cap]), cap
 
 
+def _wide_u16_chunks():
+    """u16 chunks whose materials use all 16 bits: neighbours that differ only
+    in the high byte, 0x8000 and 0xFFFF, runs ending on the 8-voxel piece
+    boundaries and at the chunk's last row and slice."""
+    rng = np.random.default_rng(77)
+    pool = np.array([0, 1, 0x00FF, 0x0100, 0x0101, 0x7FFF, 0x8000, 0x8001, 0xFFFF], np.uint16)
+    out = []
+    a = pool[rng.integers(0, len(pool), 32768)]  # independent voxels
+    out.append(a)
+    b = np.repeat(pool[rng.integers(0, len(pool), 32768 // 4)], 4)  # runs of 4 along x
+    out.append(b)
+    c = np.zeros((32, 32, 32), np.uint16)  # (z, y, x): stripes differing in the high byte only
+    c[:, :, 0::2] = 0x0001
+    c[:, :, 1::2] = 0x0101
+    c[::3, :, :] = 0x8001
+    c[:, 31, :] = 0xFFFF
+    out.append(c.ravel())
+    d = np.full((32, 32, 32), 0xFFFF, np.uint16)  # solid, one material, holes at the edges
+    d[31, :, 7::8] = 0
+    d[:, 31, 8::8] = 0x7FFF
+    out.append(d.ravel())
+    e = np.zeros((32, 32, 32), np.uint16)  # blocks whose material varies along y and z only
+    for z in range(32):
+        for y in range(32):
+            e[z, y, (y + z) % 8:(y + z) % 8 + 20] = 0x0100 * ((y // 4) % 2) + 0x0001 * ((z // 8) % 2) + 1
+    out.append(e.ravel())
+    f = pool[rng.integers(0, len(pool), 32768)] * (rng.random(32768) < 0.15)  # sparse
+    out.append(f.astype(np.uint16))
+    return np.stack(out)
+
+
+@pytest.mark.parametrize("mode", [L.MESH_CULLED, L.MESH_GREEDY])
+def test_wide_u16_materials_match_oracle(ctx, mode):
+    """The row masks of u16 chunks compare whole 16-bit materials: the batched
+    and the per-chunk path against the oracle on materials that differ only in
+    their high byte or use the top bit."""
+    chunks = _wide_u16_chunks()
+    n = chunks.shape[0]
+    refs = [O.mesh_chunk(chunks[i], mode == L.MESH_GREEDY) for i in range(n)]
+    total = sum(len(r) for r in refs)
+    buf = ctx.upload(chunks.ravel())
+    mt = R.MeshTargets(ctx, n, total)
+    R.mesh_chunks(ctx, R.chunk_source(buf, 2, n), mode, mt)
+    ctx.sync()
+    counts, prefix, tot, quads = mt.results()
+    assert tot == total
+    cnt = ctx.buffer(4)
+    q = ctx.buffer(4 * 98304)
+    for i in range(n):
+        assert counts[i] == len(refs[i]), i
+        assert np.array_equal(quads[prefix[i]:prefix[i] + counts[i]], refs[i]), i
+        R.mesh_chunk(ctx, buf.ptr + i * 65536, 2, mode, cnt.ptr, q.ptr, 98304)
+        ctx.sync()
+        k = int(cnt.download(np.uint32)[0])
+        assert k == len(refs[i]) and np.array_equal(q.download(np.uint32, k), refs[i]), i
+
+
 def test_graph_replay_matches_eager(ctx):
     n = 128
     vox = ctx.buffer(2 * 32768 * n)
