@@ -89,7 +89,8 @@ meso_status meso_device_info_get(meso_context *ctx, meso_device_info *out);
 meso_status meso_buffer_create(meso_context *ctx, uint64_t size, const void *init,
                                meso_buffer **out);
 meso_status meso_buffer_destroy(meso_buffer *buf);
-/* out-of-range copies fail with ARGUMENT_OUT_OF_RANGE (VulkanClasses.cpp:3709-3711) */
+/* out-of-range copies fail with ARGUMENT_OUT_OF_RANGE (VulkanClasses.cpp:3709-3711); a copy
+ * runs on the context stream after the work queued there and is finished on return */
 meso_status meso_buffer_upload(meso_buffer *buf, const void *src, uint64_t size, uint64_t offset);
 meso_status meso_buffer_download(meso_buffer *buf, void *dst, uint64_t size, uint64_t offset);
 /* device address of buf+offset; offset must be 8-byte aligned and inside the

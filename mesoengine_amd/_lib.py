@@ -228,4 +228,9 @@ def check(status):
 
 
 def byref(s):
-    return C.cast(C.byref(s), C.c_void_p)
+    """Pointer argument to a struct for a void-pointer parameter.  The
+    returned object holds a reference to `s`, so a temporary passed as
+    byref(make_struct()) lives until the call returns.  (A c_void_p made by
+    casting holds none: the struct could be freed, and its first bytes
+    reused by the allocator, before the library read it.)"""
+    return C.byref(s)

@@ -73,10 +73,14 @@ meso_status band_queue(meso_context *ctx, hipStream_t stream, unsigned int **out
   }
   unsigned int *q = nullptr;
   MESO_HIP(hipMalloc((void **)&q, 1024));
-  const hipError_t e = hipMemset(q, 0, 1024);
-  if (e != hipSuccess) {
+  // cleared on the stream that uses it: a hipMemset runs on the null stream,
+  // which the context's non-blocking streams do not wait for, so the first
+  // launch could read the counters before they are zero (and a clear landing
+  // during that launch leaves them past the end for every later launch)
+  const meso_status st = launch_zero(q, 1024, stream);
+  if (st != MESO_OK) {
     hipFree(q);
-    return fail(MESO_ERR_RUNTIME, "hipMemset: %s", hipGetErrorString(e));
+    return st;
   }
   ctx->band_queues[stream] = q;
   *out = q;
@@ -499,7 +503,8 @@ meso_status meso_buffer_create(meso_context *ctx, uint64_t size, const void *ini
   void *p = nullptr;
   MESO_HIP(hipMalloc(&p, size));
   if (init) {
-    hipError_t e = hipMemcpy(p, init, size, hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpyAsync(p, init, size, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
       hipFree(p);
       return fail(MESO_ERR_RUNTIME, "initial upload failed: %s", hipGetErrorString(e));
@@ -526,7 +531,11 @@ meso_status meso_buffer_upload(meso_buffer *buf, const void *src, uint64_t size,
   MESO_CHECK(offset <= buf->size && size <= buf->size - offset, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
              "upload [%llu,+%llu) outside buffer of %llu bytes", (unsigned long long)offset,
              (unsigned long long)size, (unsigned long long)buf->size);
-  MESO_HIP(hipMemcpy((char *)buf->ptr + offset, src, size, hipMemcpyHostToDevice));
+  // on the context stream: after the work queued there that may still read
+  // the buffer, and finished before the call returns
+  MESO_HIP(hipMemcpyAsync((char *)buf->ptr + offset, src, size, hipMemcpyHostToDevice,
+                          buf->ctx->stream));
+  MESO_HIP(hipStreamSynchronize(buf->ctx->stream));
   return MESO_OK;
 }
 
@@ -535,7 +544,10 @@ meso_status meso_buffer_download(meso_buffer *buf, void *dst, uint64_t size, uin
   MESO_CHECK(offset <= buf->size && size <= buf->size - offset, MESO_ERR_ARGUMENT_OUT_OF_RANGE,
              "download [%llu,+%llu) outside buffer of %llu bytes", (unsigned long long)offset,
              (unsigned long long)size, (unsigned long long)buf->size);
-  MESO_HIP(hipMemcpy(dst, (const char *)buf->ptr + offset, size, hipMemcpyDeviceToHost));
+  // on the context stream: after the work queued there that writes the buffer
+  MESO_HIP(hipMemcpyAsync(dst, (const char *)buf->ptr + offset, size, hipMemcpyDeviceToHost,
+                          buf->ctx->stream));
+  MESO_HIP(hipStreamSynchronize(buf->ctx->stream));
   return MESO_OK;
 }
 

@@ -76,3 +76,14 @@ def test_engine_instance_builds_and_rejects_bad_arguments():
     p = subprocess.run([sb, os.path.join(b, "no_such_requests.bin")], capture_output=True,
                        text=True, timeout=60)
     assert p.returncode == 1 and "cannot read" in p.stderr
+
+
+def test_byref_keeps_a_temporary_struct_alive():
+    # byref(make_struct()) is passed straight into calls: the pointer object
+    # must hold the struct, or the allocator may reuse its bytes (the first
+    # 8 of a freed block hold the free list) before the library reads them
+    p = L.byref(L.Shading(C.sizeof(L.Shading), (C.c_float * 3)(0.4, 1.0, 0.3), 0, 4, 16.0))
+    held = p._obj
+    assert isinstance(held, L.Shading)
+    assert list(held.light) == [C.c_float(0.4).value, 1.0, C.c_float(0.3).value]
+    assert C.addressof(held) == C.cast(p, C.c_void_p).value

@@ -213,6 +213,49 @@ def test_autotuners_keep_results(ctx, tmp_path):
     sc.destroy()
 
 
+def test_persistent_shapes_write_every_pixel_after_the_tuner():
+    """A fresh context (its band counters allocated where a freed buffer held
+    other bytes) tunes a frame, then renders it in every persistent shape into
+    poisoned outputs: every pixel is written and equals the oracle's."""
+    c2 = R.Context(0)
+    try:
+        _render_after_tuner(c2)  # its buffers are freed on return, before the context
+    finally:
+        c2.close()
+
+
+def _render_after_tuner(c2):
+    junk = c2.buffer(16 << 20, np.full(16 << 20, 0xFF, np.uint8))
+    del junk
+    sc = c2.scene_generate(L.GEN_SPHERES, L.SCENE_DENSE_U8, 128, 1, 8)
+    cp = W.spheres_camera(width=200, height=120)
+    cam = R.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], 200, 120)
+    ocam = O.camera(cp["eye"], cp["target"], cp["up"], cp["vfov"], 200, 120)
+    ref = O.raymarch(O.dense_scene(sc.arrays()["dense"]), ocam, O.RAY_SHADOW, O.shading())
+    tg = R.RayTargets(c2, 200, 120)
+    rr = L.TuneResult()
+    rr.size = C.sizeof(rr)
+    L.check(c2.lib.meso_autotune_raymarch(c2.h, sc.h, L.byref(cam), L.RAY_SHADOW,
+                                          L.byref(R.shading()), L.byref(tg.out), L.byref(rr)))
+    cus = c2.device_info().compute_units
+    tiles = {5: (8, 8), 6: (16, 4), 7: (32, 2), 8: (8, 8), 9: (16, 4)}
+    for v, g in ((5, 0), (5, 16 * cus), (6, 0), (7, 0), (8, 0), (8, 16 * cus), (9, 0)):
+        pin = L.TuneResult()
+        pin.size = C.sizeof(pin)
+        pin.variant, pin.grid = v, g
+        pin.tile_w, pin.tile_h = tiles[v]
+        L.check(c2.lib.meso_autotune_pin_raymarch(c2.h, sc.h, L.byref(cam), L.RAY_SHADOW,
+                                                  L.byref(pin)))
+        for b in (tg.index, tg.face, tg.t, tg.shade):
+            b.upload(np.full(b.nbytes, 0xA5, np.uint8))
+        R.raymarch(c2, sc, cam, L.RAY_SHADOW, R.shading(), tg)
+        c2.sync()
+        for a, b in zip(tg.results(), ref[:4]):
+            b = np.ascontiguousarray(np.asarray(b, a.dtype))
+            assert np.array_equal(np.ascontiguousarray(a).view(np.uint8), b.view(np.uint8)), (v, g)
+    sc.destroy()
+
+
 def test_reference_sphere_chunks_and_erosion(ctx):
     coords = np.array([(cx, cy, cz) for cz in range(-4, 4) for cy in range(-4, 4)
                        for cx in range(3, 10)], np.int32)
